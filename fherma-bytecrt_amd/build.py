@@ -1,0 +1,62 @@
+"""Build recipe for the gfx950 ByteCRT library (hipcc, no cmake).
+
+    python fherma-bytecrt_amd/build.py
+
+Outputs (git-ignored, shipped in-tree to the GPU host):
+    fherma-bytecrt_amd/lib/libbytecrt.so   C ABI (include/bytecrt.h) + fherma.h wrapper
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "lib"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_engine.cpp", "fherma_solve.cpp"]
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build failed: {' '.join(cmd[:3])} ...")
+    return r
+
+
+def build(verbose: bool = False) -> Path:
+    LIB.mkdir(exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        s = CSRC / src
+        o = LIB / (s.stem + ".o")
+        cmd = [HIPCC, *COMMON, "-c", str(s), "-o", str(o)]
+        if s.suffix == ".hip":
+            cmd[1:1] = DEVICE
+        else:
+            cmd[1:1] = [f"--offload-arch={ARCH}", "-x", "hip"] if src == "bytecrt_engine.cpp" else []
+        _run(cmd)
+        objs.append(str(o))
+    out = LIB / "libbytecrt.so"
+    tmp = LIB / "libbytecrt.so.tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs])
+    os.replace(tmp, out)
+    for o in objs:
+        os.remove(o)
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,279 @@
+"""Python host mirror of the ByteCRT boundary over the gfx950 C ABI.
+
+The reference boundary is three C++ calls (native-batched/fherma.h:47-49):
+``fherma_init(Point)``, ``fherma_run(state, Inputs) -> Outputs`` and
+``fherma_free(state)``.  This module keeps those names, argument meanings and
+error behaviour (``RuntimeError`` where the reference throws
+``std::runtime_error``, ``ValueError`` for ``std::invalid_argument``; messages
+start with "ByteCRT") over numpy arrays, and adds the device-resident entry
+points of include/bytecrt.h for torch tensors already in HBM.
+
+There is no CPU fallback: if libbytecrt.so is missing or no GPU is present,
+every compute call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libbytecrt.so"
+
+BCRT_OK, BCRT_EINVAL, BCRT_ERUNTIME = 0, 1, 2
+STAGE_ENCODE, STAGE_FWD_COLS, STAGE_MIDDLE, STAGE_INV_COLS, STAGE_CRT = 1, 2, 4, 8, 16
+STAGE_ALL = 31
+
+EXPORTED = (
+    "bcrt_create", "bcrt_destroy", "bcrt_run", "bcrt_run_device", "bcrt_run_device_graph",
+    "bcrt_synchronize", "bcrt_last_error", "bcrt_geometry", "bcrt_plan_tables",
+    "bcrt_run_stages", "bcrt_planes_words", "bcrt_mfma_probe", "bcrt_plan_host",
+)
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_LIB = None
+
+
+def load_library():
+    """Load libbytecrt.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"ByteCRT native library missing: {LIB_PATH} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp = ctypes.c_void_p
+    lib.bcrt_create.restype = ctypes.c_int
+    lib.bcrt_create.argtypes = [ctypes.POINTER(vp), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_uint32, _u32p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.bcrt_destroy.argtypes = [vp]
+    lib.bcrt_run.restype = ctypes.c_int
+    lib.bcrt_run.argtypes = [vp, _u32p, _u32p, _u32p, ctypes.c_uint32]
+    for name in ("bcrt_run_device", "bcrt_run_device_graph"):
+        f = getattr(lib, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [vp, vp, vp, vp, ctypes.c_uint32, vp]
+    lib.bcrt_synchronize.restype = ctypes.c_int
+    lib.bcrt_synchronize.argtypes = [vp]
+    lib.bcrt_last_error.restype = ctypes.c_char_p
+    lib.bcrt_last_error.argtypes = [vp]
+    lib.bcrt_geometry.restype = ctypes.c_int
+    lib.bcrt_geometry.argtypes = [vp, _u32p]
+    lib.bcrt_plan_tables.restype = ctypes.c_int
+    lib.bcrt_plan_tables.argtypes = [vp, _u32p, _u32p, _u32p, _u32p]
+    lib.bcrt_run_stages.restype = ctypes.c_int
+    lib.bcrt_run_stages.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, vp, ctypes.c_uint32, vp]
+    lib.bcrt_planes_words.restype = ctypes.c_size_t
+    lib.bcrt_planes_words.argtypes = [vp, ctypes.c_uint32]
+    lib.bcrt_mfma_probe.restype = ctypes.c_int
+    lib.bcrt_mfma_probe.argtypes = [vp, vp, vp]
+    lib.bcrt_plan_host.restype = ctypes.c_int
+    lib.bcrt_plan_host.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p,
+                                   _u32p, _u32p, _u32p, _u32p, ctypes.c_char_p, ctypes.c_size_t]
+    _LIB = lib
+    return lib
+
+
+def _raise(code: int, msg: str):
+    if code == BCRT_EINVAL:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def limbs_of(q: int, L: int) -> np.ndarray:
+    return np.array([(q >> (32 * i)) & 0xFFFFFFFF for i in range(L)], dtype=np.uint32)
+
+
+def _dptr(t) -> int:
+    """Device address of a torch tensor (or an int address)."""
+    if isinstance(t, int):
+        return t
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("ByteCRT device entry points need contiguous GPU tensors")
+    return t.data_ptr()
+
+
+def _stream_handle(stream) -> int | None:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class Engine:
+    """One plan (N, W, q) with device workspace for up to ``max_batch`` products."""
+
+    def __init__(self, N: int, W: int, q: int, max_batch: int = 1):
+        lib = load_library()
+        self.N, self.W, self.q, self.max_batch = N, W, q, max_batch
+        self.L = (W + 31) // 32
+        qw = limbs_of(q, self.L)
+        handle = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = lib.bcrt_create(ctypes.byref(handle), max_batch, N, W, self.L,
+                             qw.ctypes.data_as(_u32p), err, 512)
+        if rc != BCRT_OK:
+            _raise(rc, err.value.decode())
+        self._h = handle
+        g = np.zeros(10, dtype=np.uint32)
+        lib.bcrt_geometry(self._h, g.ctypes.data_as(_u32p))
+        (self.N, self.W, self.L, self.P, self.N1, self.N2, self.Kenc, self.Kcrt, self.Mcrt,
+         self.product_bits) = (int(v) for v in g)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().bcrt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def _check(self, rc: int):
+        if rc != BCRT_OK:
+            _raise(rc, load_library().bcrt_last_error(self._h).decode())
+
+    # -- plan ------------------------------------------------------------------
+    def plan_tables(self):
+        P, L = self.P, self.L
+        primes, psi, crt = (np.zeros(P, dtype=np.uint32) for _ in range(3))
+        weights = np.zeros((P + 1, L), dtype=np.uint32)
+        self._check(load_library().bcrt_plan_tables(
+            self._h, primes.ctypes.data_as(_u32p), psi.ctypes.data_as(_u32p),
+            crt.ctypes.data_as(_u32p), weights.ctypes.data_as(_u32p)))
+        return dict(primes=primes, psi=psi, crt_inverse=crt, weights=weights)
+
+    def planes_words(self, batch: int) -> int:
+        return int(load_library().bcrt_planes_words(self._h, batch))
+
+    # -- compute ---------------------------------------------------------------
+    def run(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        """Whole call on host arrays [B][N][L] (or [N][L])."""
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        if a.shape != b.shape or a.size % (self.N * self.L):
+            raise RuntimeError("ByteCRT input coefficient count")
+        batch = a.size // (self.N * self.L)
+        c = np.empty_like(a)
+        self._check(load_library().bcrt_run(self._h, a.ctypes.data_as(_u32p), b.ctypes.data_as(_u32p),
+                                            c.ctypes.data_as(_u32p), batch))
+        return c
+
+    def run_device(self, a, b, c, batch: int, stream=None, graph: bool = False):
+        """Enqueue on device tensors (no sync).  graph=True replays a captured hipGraph."""
+        f = load_library().bcrt_run_device_graph if graph else load_library().bcrt_run_device
+        self._check(f(self._h, _dptr(a), _dptr(b), _dptr(c), batch, _stream_handle(stream)))
+
+    def run_stages(self, mask: int, a=None, b=None, c=None, planes=None, batch: int = 1, stream=None):
+        lib = load_library()
+        ptr = (lambda t: None if t is None else _dptr(t))
+        self._check(lib.bcrt_run_stages(self._h, mask, ptr(a), ptr(b), ptr(c), ptr(planes), batch,
+                                        _stream_handle(stream)))
+
+    def synchronize(self):
+        self._check(load_library().bcrt_synchronize(self._h))
+
+
+def plan_host(N: int, W: int, q: int) -> dict:
+    """Host-only plan of the product library (no GPU needed)."""
+    lib = load_library()
+    L = (W + 31) // 32
+    qw = limbs_of(q, L)
+    geo = np.zeros(10, dtype=np.uint32)
+    err = ctypes.create_string_buffer(512)
+    rc = lib.bcrt_plan_host(N, W, L, qw.ctypes.data_as(_u32p), geo.ctypes.data_as(_u32p),
+                            None, None, None, None, err, 512)
+    if rc != BCRT_OK:
+        _raise(rc, err.value.decode())
+    P = int(geo[3])
+    primes, psi, crt = (np.zeros(P, dtype=np.uint32) for _ in range(3))
+    weights = np.zeros((P + 1, L), dtype=np.uint32)
+    rc = lib.bcrt_plan_host(N, W, L, qw.ctypes.data_as(_u32p), geo.ctypes.data_as(_u32p),
+                            primes.ctypes.data_as(_u32p), psi.ctypes.data_as(_u32p),
+                            crt.ctypes.data_as(_u32p), weights.ctypes.data_as(_u32p), err, 512)
+    if rc != BCRT_OK:
+        _raise(rc, err.value.decode())
+    keys = ("N", "W", "L", "P", "N1", "N2", "Kenc", "Kcrt", "Mcrt", "product_bits")
+    out = {k: int(v) for k, v in zip(keys, geo)}
+    out.update(primes=primes, psi=psi, crt_inverse=crt, weights=weights)
+    return out
+
+
+def mfma_probe(a, b, c):
+    rc = load_library().bcrt_mfma_probe(_dptr(a), _dptr(b), _dptr(c))
+    if rc != BCRT_OK:
+        _raise(rc, "ByteCRT MFMA probe failed")
+
+
+# ---------------------------------------------------------------------------
+# fherma.h mirror
+# ---------------------------------------------------------------------------
+@dataclass
+class Tensor:
+    shape: list = field(default_factory=list)
+    data: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.uint32))
+
+    def count(self) -> int:
+        n = 1
+        for d in self.shape:
+            n *= int(d)
+        return n
+
+
+@dataclass
+class Point:
+    B: int = 0
+    N: int = 0
+    W: int = 0
+    L: int = 0
+    q: Tensor = field(default_factory=Tensor)
+
+
+@dataclass
+class Inputs:
+    a: Tensor = field(default_factory=Tensor)
+    b: Tensor = field(default_factory=Tensor)
+
+
+@dataclass
+class Outputs:
+    c: Tensor = field(default_factory=Tensor)
+
+
+class _State:
+    def __init__(self, engine: Engine, B: int):
+        self.engine, self.B = engine, B
+
+
+def fherma_init(p: Point) -> _State:
+    """native-batched/solve.cu:160-198 semantics."""
+    if p.B == 0 or p.B > 128:
+        raise RuntimeError("ByteCRT batch coverage B=1..128")
+    qd = np.asarray(p.q.data, dtype=np.uint32)
+    if qd.size != p.L:
+        raise ValueError("ByteCRT RNS modulus limb count")
+    q = int.from_bytes(qd.astype("<u4").tobytes(), "little")
+    if p.L != (p.W + 31) // 32:
+        raise ValueError("ByteCRT point L mismatch")
+    return _State(Engine(p.N, p.W, q, max_batch=p.B), p.B)
+
+
+def fherma_run(state: _State, inputs: Inputs) -> Outputs:
+    """native-batched/solve.cu:200-244 semantics (whole call)."""
+    e = state.engine
+    total = state.B * e.N * e.L
+    a = np.asarray(inputs.a.data, dtype=np.uint32).reshape(-1)
+    b = np.asarray(inputs.b.data, dtype=np.uint32).reshape(-1)
+    if a.size != total or b.size != total:
+        raise RuntimeError("ByteCRT input coefficient count")
+    c = e.run(a.reshape(state.B, e.N, e.L), b.reshape(state.B, e.N, e.L))
+    return Outputs(c=Tensor(shape=[state.B, e.N, e.L], data=c.reshape(-1)))
+
+
+def fherma_free(state: _State) -> None:
+    state.engine.close()

@@ -1,0 +1,318 @@
+// C ABI of the ByteCRT engine: plan upload, workspace, whole-call and
+// device-resident runs, graph replay.
+#include "../../include/bytecrt.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bytecrt_kernels.hpp"
+#include "bytecrt_plan.hpp"
+
+namespace bcrt {
+hipError_t configure_kernels();
+}
+
+struct bcrt_engine {
+    bcrt::HostPlan plan;
+    bcrt::DevTables tables{};
+    uint32_t max_batch = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocations;
+    uint32_t* planes = nullptr;  // [max_batch][2][P][N]
+    // Whole-call staging (lazily allocated).
+    uint32_t* io = nullptr;      // [3][max_batch][N][L]
+    // Captured graph (one key).
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t graph_batch = 0;
+    std::string last_error;
+};
+
+namespace {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string("ByteCRT ") + what + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* upload(bcrt_engine& s, const std::vector<T>& v) {
+    void* p = nullptr;
+    check(hipMalloc(&p, v.size() * sizeof(T)), "allocate table");
+    s.allocations.push_back(p);
+    check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload table");
+    return static_cast<T*>(p);
+}
+
+int fail(bcrt_engine* e, int code, const std::string& msg) {
+    if (e) e->last_error = msg;
+    return code;
+}
+
+// Runs f, mapping exceptions to status codes.
+template <class F>
+int guarded(bcrt_engine* e, F&& f) {
+    try {
+        f();
+        if (e) e->last_error.clear();
+        return BCRT_OK;
+    } catch (const std::invalid_argument& x) {
+        return fail(e, BCRT_EINVAL, x.what());
+    } catch (const std::exception& x) {
+        return fail(e, BCRT_ERUNTIME, x.what());
+    }
+}
+
+hipStream_t pick(bcrt_engine* e, void* stream) {
+    return stream ? static_cast<hipStream_t>(stream) : e->stream;
+}
+
+void require_batch(const bcrt_engine* e, uint32_t batch) {
+    if (batch == 0 || batch > e->max_batch)
+        throw std::invalid_argument("ByteCRT batch must be in [1, max_batch=" +
+                                    std::to_string(e->max_batch) + "]");
+}
+
+void drop_graph(bcrt_engine* e) {
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->graph_exec = nullptr;
+    e->graph = nullptr;
+    e->graph_batch = 0;
+}
+
+void release(bcrt_engine* e) {
+    drop_graph(e);
+    for (void* p : e->allocations) (void)hipFree(p);
+    e->allocations.clear();
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    e->stream = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcrt_create(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, uint32_t L,
+                const uint32_t* q, char* err, size_t errlen) {
+    if (!out) return BCRT_EINVAL;
+    *out = nullptr;
+    auto* e = new bcrt_engine();
+    const int rc = guarded(e, [&] {
+        if (max_batch == 0) throw std::invalid_argument("ByteCRT max_batch must be positive");
+        e->plan = bcrt::make_plan(N, W, L, q);
+        const auto& g = e->plan.g;
+        e->max_batch = max_batch;
+        check(hipGetDevice(&e->device), "get device");
+        check(bcrt::configure_kernels(), "configure kernels");
+        // A blocking stream: work enqueued here is ordered with the legacy
+        // default stream, which is where callers' allocations and copies land.
+        check(hipStreamCreate(&e->stream), "create stream");
+        bcrt::DevTables& t = e->tables;
+        t.N = g.N;
+        t.W = g.W;
+        t.L = g.L;
+        t.P = g.P;
+        t.N1 = g.N1;
+        t.n1 = g.n1;
+        t.N2 = g.N2;
+        t.Kenc = g.Kenc;
+        t.PTenc = g.PTenc;
+        t.Kcrt = g.Kcrt;
+        t.Mcrt = g.Mcrt;
+        t.qshift = e->plan.qshift;
+        t.inv_qt1 = e->plan.inv_qt1;
+        t.primes = upload(*e, e->plan.primes);
+        t.mont_neg_inv = upload(*e, e->plan.mont_neg_inv);
+        t.zf = reinterpret_cast<const uint2*>(upload(*e, e->plan.zf));
+        t.zi = reinterpret_cast<const uint2*>(upload(*e, e->plan.zi));
+        t.scale = reinterpret_cast<const uint2*>(upload(*e, e->plan.scale));
+        t.wenc = upload(*e, e->plan.wenc);
+        t.enc_off = upload(*e, e->plan.enc_off);
+        t.enc_mu = upload(*e, e->plan.enc_mu);
+        t.wcrt = upload(*e, e->plan.wcrt);
+        t.corr = upload(*e, e->plan.corr);
+        t.invp = upload(*e, e->plan.invp);
+        t.q = upload(*e, e->plan.q);
+        // The host copies of the big tables are no longer needed.
+        std::vector<uint32_t>().swap(e->plan.zf);
+        std::vector<uint32_t>().swap(e->plan.zi);
+        void* p = nullptr;
+        check(hipMalloc(&p, size_t(max_batch) * 2 * g.P * g.N * sizeof(uint32_t)), "allocate residue planes");
+        e->allocations.push_back(p);
+        e->planes = static_cast<uint32_t*>(p);
+    });
+    if (rc != BCRT_OK) {
+        if (err && errlen) {
+            std::snprintf(err, errlen, "%s", e->last_error.c_str());
+        }
+        release(e);
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return BCRT_OK;
+}
+
+void bcrt_destroy(bcrt_engine* e) {
+    if (!e) return;
+    release(e);
+    delete e;
+}
+
+int bcrt_run_device(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
+                    uint32_t batch, void* stream) {
+    if (!e) return BCRT_EINVAL;
+    return guarded(e, [&] {
+        require_batch(e, batch);
+        if (!d_a || !d_b || !d_c) throw std::invalid_argument("ByteCRT null device pointer");
+        const bcrt::BatchArgs args{d_a, d_b, d_c, e->planes, batch};
+        check(bcrt::launch_product(e->tables, args, pick(e, stream)), "launch product");
+    });
+}
+
+int bcrt_run_device_graph(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
+                          uint32_t batch, void* stream) {
+    if (!e) return BCRT_EINVAL;
+    return guarded(e, [&] {
+        require_batch(e, batch);
+        if (!d_a || !d_b || !d_c) throw std::invalid_argument("ByteCRT null device pointer");
+        hipStream_t s = pick(e, stream);
+        const bool hit = e->graph_exec && e->graph_batch == batch && e->graph_key[0] == d_a &&
+                         e->graph_key[1] == d_b && e->graph_key[2] == d_c;
+        if (!hit) {
+            drop_graph(e);
+            // Capture on the engine's own stream, then launch on the caller's.
+            check(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+            const bcrt::BatchArgs args{d_a, d_b, d_c, e->planes, batch};
+            const hipError_t le = bcrt::launch_product(e->tables, args, e->stream);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+            check(le, "launch product (capture)");
+            check(ce, "end capture");
+            e->graph = g;
+            check(hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0), "instantiate graph");
+            e->graph_key[0] = d_a;
+            e->graph_key[1] = d_b;
+            e->graph_key[2] = d_c;
+            e->graph_batch = batch;
+        }
+        check(hipGraphLaunch(e->graph_exec, s), "graph launch");
+    });
+}
+
+int bcrt_run(bcrt_engine* e, const uint32_t* a, const uint32_t* b, uint32_t* c, uint32_t batch) {
+    if (!e) return BCRT_EINVAL;
+    return guarded(e, [&] {
+        require_batch(e, batch);
+        if (!a || !b || !c) throw std::invalid_argument("ByteCRT null host pointer");
+        const auto& g = e->plan.g;
+        const size_t words = size_t(g.N) * g.L;
+        if (!e->io) {
+            void* p = nullptr;
+            check(hipMalloc(&p, 3 * size_t(e->max_batch) * words * sizeof(uint32_t)), "allocate io buffers");
+            e->allocations.push_back(p);
+            e->io = static_cast<uint32_t*>(p);
+        }
+        uint32_t* da = e->io;
+        uint32_t* db = da + size_t(e->max_batch) * words;
+        uint32_t* dc = db + size_t(e->max_batch) * words;
+        const size_t bytes = size_t(batch) * words * sizeof(uint32_t);
+        check(hipMemcpyAsync(da, a, bytes, hipMemcpyHostToDevice, e->stream), "copy a");
+        check(hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, e->stream), "copy b");
+        const bcrt::BatchArgs args{da, db, dc, e->planes, batch};
+        check(bcrt::launch_product(e->tables, args, e->stream), "launch product");
+        check(hipMemcpyAsync(c, dc, bytes, hipMemcpyDeviceToHost, e->stream), "copy c");
+        check(hipStreamSynchronize(e->stream), "synchronize");
+    });
+}
+
+int bcrt_synchronize(bcrt_engine* e) {
+    if (!e) return BCRT_EINVAL;
+    return guarded(e, [&] { check(hipStreamSynchronize(e->stream), "synchronize"); });
+}
+
+const char* bcrt_last_error(const bcrt_engine* e) { return e ? e->last_error.c_str() : "null engine"; }
+
+int bcrt_geometry(const bcrt_engine* e, uint32_t* out) {
+    if (!e || !out) return BCRT_EINVAL;
+    const auto& g = e->plan.g;
+    const uint32_t v[10] = {g.N, g.W, g.L, g.P, g.N1, g.N2, g.Kenc, g.Kcrt, g.Mcrt, g.product_bits};
+    std::memcpy(out, v, sizeof(v));
+    return BCRT_OK;
+}
+
+int bcrt_plan_tables(const bcrt_engine* e, uint32_t* primes, uint32_t* psi, uint32_t* crt_inverse,
+                     uint32_t* weights) {
+    if (!e) return BCRT_EINVAL;
+    const auto& pl = e->plan;
+    if (primes) std::memcpy(primes, pl.primes.data(), pl.primes.size() * 4);
+    if (psi) std::memcpy(psi, pl.psi.data(), pl.psi.size() * 4);
+    if (crt_inverse) std::memcpy(crt_inverse, pl.crt_inverse.data(), pl.crt_inverse.size() * 4);
+    if (weights) std::memcpy(weights, pl.weights.data(), pl.weights.size() * 4);
+    return BCRT_OK;
+}
+
+int bcrt_run_stages(bcrt_engine* e, uint32_t stage_mask, const uint32_t* d_a, const uint32_t* d_b,
+                    uint32_t* d_c, uint32_t* d_planes, uint32_t batch, void* stream) {
+    if (!e) return BCRT_EINVAL;
+    return guarded(e, [&] {
+        require_batch(e, batch);
+        if ((stage_mask & 1) && (!d_a || !d_b)) throw std::invalid_argument("ByteCRT encode needs a and b");
+        if ((stage_mask & 16) && !d_c) throw std::invalid_argument("ByteCRT CRT needs c");
+        const bcrt::BatchArgs args{d_a, d_b, d_c, d_planes ? d_planes : e->planes, batch};
+        hipStream_t s = pick(e, stream);
+        if (stage_mask & 1) check(bcrt::launch_encode(e->tables, args, s), "encode");
+        if (stage_mask & 2) check(bcrt::launch_forward_columns(e->tables, args, s), "forward columns");
+        if (stage_mask & 4) check(bcrt::launch_middle(e->tables, args, s), "middle");
+        if (stage_mask & 8) check(bcrt::launch_inverse_columns(e->tables, args, s), "inverse columns");
+        if (stage_mask & 16) check(bcrt::launch_crt(e->tables, args, s), "crt");
+    });
+}
+
+size_t bcrt_planes_words(const bcrt_engine* e, uint32_t batch) {
+    return e ? size_t(2) * e->plan.g.P * e->plan.g.N * batch : 0;
+}
+
+int bcrt_mfma_probe(const int8_t* d_a, const int8_t* d_b, int32_t* d_c) {
+    if (!d_a || !d_b || !d_c) return BCRT_EINVAL;
+    if (bcrt::launch_mfma_probe(d_a, d_b, d_c, nullptr) != hipSuccess) return BCRT_ERUNTIME;
+    return hipDeviceSynchronize() == hipSuccess ? BCRT_OK : BCRT_ERUNTIME;
+}
+
+}  // extern "C"
+
+extern "C" int bcrt_plan_host(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, uint32_t* geometry,
+                              uint32_t* primes, uint32_t* psi, uint32_t* crt_inverse, uint32_t* weights,
+                              char* err, size_t errlen) {
+    try {
+        const bcrt::HostPlan pl = bcrt::make_plan(N, W, L, q);
+        const auto& g = pl.g;
+        if (geometry) {
+            const uint32_t v[10] = {g.N, g.W, g.L, g.P, g.N1, g.N2, g.Kenc, g.Kcrt, g.Mcrt, g.product_bits};
+            std::memcpy(geometry, v, sizeof(v));
+        }
+        if (primes) std::memcpy(primes, pl.primes.data(), pl.primes.size() * 4);
+        if (psi) std::memcpy(psi, pl.psi.data(), pl.psi.size() * 4);
+        if (crt_inverse) std::memcpy(crt_inverse, pl.crt_inverse.data(), pl.crt_inverse.size() * 4);
+        if (weights) std::memcpy(weights, pl.weights.data(), pl.weights.size() * 4);
+        return BCRT_OK;
+    } catch (const std::invalid_argument& x) {
+        if (err && errlen) std::snprintf(err, errlen, "%s", x.what());
+        return BCRT_EINVAL;
+    } catch (const std::exception& x) {
+        if (err && errlen) std::snprintf(err, errlen, "%s", x.what());
+        return BCRT_ERUNTIME;
+    }
+}

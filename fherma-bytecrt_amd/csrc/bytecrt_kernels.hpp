@@ -1,0 +1,54 @@
+// Kernel launch interface of the MI355X ByteCRT engine (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bcrt {
+
+// Device-resident constant tables of one plan, passed by value to every kernel.
+struct DevTables {
+    uint32_t N, W, L, P, N1, n1, N2, Kenc, PTenc, Kcrt, Mcrt, qshift;
+    double inv_qt1;
+    const uint32_t* primes;        // [P]
+    const uint32_t* mont_neg_inv;  // [P]
+    const uint2* zf;               // [P][N] (w, Shoup w')
+    const uint2* zi;               // [P][N]
+    const uint2* scale;            // [P]
+    const int8_t* wenc;            // [PTenc][32][Kenc]
+    const uint64_t* enc_off;       // [P]
+    const uint64_t* enc_mu;        // [P]
+    const int8_t* wcrt;            // [Mcrt][Kcrt]
+    const int64_t* corr;           // [Mcrt/4]
+    const double* invp;            // [P]
+    const uint32_t* q;             // [L]
+};
+
+// One batch of products.  a, b, out: [batch][N][L] u32 coefficient-major.
+// planes: [batch][2][P][N] u32 workspace (operand A planes end up holding the
+// CRT terms).  product_stride_in/out are the element strides between products
+// (N*L for dense batches).
+struct BatchArgs {
+    const uint32_t* a;
+    const uint32_t* b;
+    uint32_t* out;
+    uint32_t* planes;
+    uint32_t batch;
+};
+
+// Enqueue the whole product pipeline for a batch on `stream` (no host sync).
+hipError_t launch_product(const DevTables& t, const BatchArgs& args, hipStream_t stream);
+
+// Individual stages (tests and profiling).
+hipError_t launch_encode(const DevTables& t, const BatchArgs& args, hipStream_t stream);
+hipError_t launch_forward_columns(const DevTables& t, const BatchArgs& args, hipStream_t stream);
+hipError_t launch_middle(const DevTables& t, const BatchArgs& args, hipStream_t stream);
+hipError_t launch_inverse_columns(const DevTables& t, const BatchArgs& args, hipStream_t stream);
+hipError_t launch_crt(const DevTables& t, const BatchArgs& args, hipStream_t stream);
+
+// 32x32x32 signed int8 MFMA tile: c[32][32] = a[32][32] * b[32][32]^T (b given
+// as [col][k]); one wave.  Lane-layout check for tests.
+hipError_t launch_mfma_probe(const int8_t* a, const int8_t* b, int32_t* c, hipStream_t stream);
+
+}  // namespace bcrt

@@ -1,0 +1,67 @@
+// Host-side RNS plan and device table layouts for the MI355X ByteCRT engine.
+//
+// The plan follows the reference's init-time setup (native-smoke/rns-plan.hpp:174-324,
+// reference.py:39-54): greedy descending NTT primes p == 1 mod 2N in (2^30, 2^31)
+// until bit_length(prod p) >= 2W + bit_length(N) + 2, the smallest-base primitive
+// 2N-th root psi, CRT inverses (P/p_j)^-1 mod p_j, weights P_j mod q and the alpha
+// weight (-P) mod q.  It is generalised to any q with bit_length(q) == W (the
+// reference only accepts q = 2^W - gap) and to any N = 2^k, 4096 <= N <= 65536.
+//
+// On top of that it lays out what the gfx950 kernels read:
+//  * psi-power twiddles in bit-reversed order with Shoup companions (forward and
+//    inverse merged negacyclic NTT, no twist pass);
+//  * signed int8 weight tiles for the encode and CRT MFMA GEMMs (balanced base-256
+//    digits, so the signed MFMA is exact; the unsigned data bytes are biased by
+//    -128 and a per-column constant restores the sum);
+//  * the generic-q quotient estimate constants for the CRT epilogue.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bcrt {
+
+constexpr uint32_t kMaxPrimes = 128;
+constexpr uint32_t kMaxLimbs = 64;
+constexpr uint32_t kBlockLen = 256;  // N2: contiguous block length of the second NTT pass
+
+struct Geometry {
+    uint32_t N = 0, W = 0, L = 0, P = 0;
+    uint32_t log_n = 0;
+    uint32_t N1 = 0, n1 = 0;  // column length / log2 (first NTT pass)
+    uint32_t N2 = 0;          // block length (second NTT pass)
+    uint32_t Kenc = 0;        // encode inner axis: 4L padded to 32
+    uint32_t PTenc = 0;       // encode prime tiles (8 primes each)
+    uint32_t Kcrt = 0;        // CRT inner axis: 4(P+1) padded to 32
+    uint32_t Mcrt = 0;        // CRT output digit rows: 4L+1 padded to 32
+    uint32_t product_bits = 0;
+};
+
+struct HostPlan {
+    Geometry g;
+    std::vector<uint32_t> q;            // L limbs
+    std::vector<uint32_t> primes;       // P
+    std::vector<uint32_t> psi;          // P: primitive 2N-th roots
+    std::vector<uint32_t> crt_inverse;  // P: (P/p_j)^-1 mod p_j
+    std::vector<uint32_t> weights;      // (P+1) x L: P_j mod q, last row (-P) mod q
+    std::vector<uint32_t> mont_neg_inv; // P: -p^-1 mod 2^32
+    std::vector<uint32_t> scale;        // P x 2: N^-1 * 2^32 * crt_inverse mod p, Shoup companion
+    std::vector<uint32_t> zf, zi;       // P x N x 2: (psi^{+-brv(k)}, Shoup companion)
+    std::vector<int8_t> wenc;           // PTenc x 32 x Kenc
+    std::vector<uint64_t> enc_off;      // P
+    std::vector<uint64_t> enc_mu;       // P: floor((2^64-1)/p)
+    std::vector<int8_t> wcrt;           // Mcrt x Kcrt
+    std::vector<int64_t> corr;          // Mcrt/4 limb corrections
+    std::vector<double> invp;           // P
+    uint32_t qshift = 0;                // W - 32
+    double inv_qt1 = 0;                 // 1 / ((q >> qshift) + 1)
+};
+
+// Throws std::invalid_argument (bad geometry or q) or std::runtime_error (no plan).
+HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q);
+
+// Number of primes the plan would use for (N, W) (no q needed).
+uint32_t plan_prime_count(uint32_t N, uint32_t W);
+
+}  // namespace bcrt

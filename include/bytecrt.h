@@ -1,0 +1,115 @@
+/*
+ * bytecrt.h -- C ABI of the MI355X (gfx950) ByteCRT exact negacyclic product
+ * engine: c = a * b mod (X^N + 1, q) for N wide coefficients of W bits stored
+ * as L = ceil(W/32) little-endian u32 limbs.  Plain pointers and sizes only.
+ *
+ * The reference exposes this path through three C++ entry points over
+ * std::vector tensors (native-batched/fherma.h:47-49):
+ *     void* fherma_init(const fherma::Point&);
+ *     fherma::Outputs fherma_run(void*, const fherma::Inputs&);
+ *     void fherma_free(void*);
+ * Those are provided unchanged by include/fherma.h (implemented on top of this
+ * ABI in fherma-bytecrt_amd/csrc/fherma_solve.cpp).  Every entry point below
+ * names the reference interface it replaces.
+ *
+ * Status codes: BCRT_OK, BCRT_EINVAL (what the reference raises as
+ * std::invalid_argument, and its size/batch std::runtime_error checks),
+ * BCRT_ERUNTIME (HIP / plan failures; std::runtime_error in the reference).
+ * The message of the last failure is bcrt_last_error().
+ */
+#ifndef BYTECRT_H
+#define BYTECRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BCRT_OK 0
+#define BCRT_EINVAL 1
+#define BCRT_ERUNTIME 2
+
+typedef struct bcrt_engine bcrt_engine;
+
+/* Replaces fherma_init (native-batched/fherma.h:47; native-batched/solve.cu:160-198,
+ * core/solve.cu:295-397).  Builds the RNS plan for (N, W, L, q), uploads the
+ * twiddle / MFMA weight tables and reserves device workspace for up to
+ * max_batch products.  q: L limbs, bit_length(q) == W (any such q; the
+ * reference requires q = 2^W - gap).  N = 2^k, 4096 <= N <= 65536.
+ * On failure *out is NULL and err (if given) holds the message. */
+int bcrt_create(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, uint32_t L,
+                const uint32_t* q, char* err, size_t errlen);
+
+/* Replaces fherma_free (native-batched/fherma.h:49; native-batched/solve.cu:246). */
+void bcrt_destroy(bcrt_engine* e);
+
+/* Replaces fherma_run (native-batched/fherma.h:48; native-batched/solve.cu:200-244,
+ * core/solve.cu:506-525): whole call on host buffers a, b, c of
+ * batch * N * L u32 each ([batch][N][L] row-major).  Includes the host-to-
+ * device copies, all GPU work and the device-to-host copy; returns when c is
+ * complete.  batch <= max_batch. */
+int bcrt_run(bcrt_engine* e, const uint32_t* a, const uint32_t* b, uint32_t* c, uint32_t batch);
+
+/* Device-resident form of fherma_run's GPU work (core/solve.cu:401-430 without
+ * the PCIe legs): a, b, c are device pointers ([batch][N][L] u32); the five
+ * kernels are enqueued on `stream` (a hipStream_t; NULL selects the engine's
+ * own stream) and the call returns without waiting. */
+int bcrt_run_device(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
+                    uint32_t batch, void* stream);
+
+/* Same work as bcrt_run_device, replayed from a hipGraph captured on the first
+ * call for this (pointers, batch) key (the batched adapter's stream/slot
+ * orchestration, native-batched/solve.cu:56-156, becomes one graph launch). */
+int bcrt_run_device_graph(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
+                          uint32_t batch, void* stream);
+
+/* Waits for all work the engine enqueued on its own stream. */
+int bcrt_synchronize(bcrt_engine* e);
+
+/* Message of the last failing call on this engine ("" if none). */
+const char* bcrt_last_error(const bcrt_engine* e);
+
+/* Geometry: out[0..9] = N, W, L, P, N1, N2, Kenc, Kcrt, Mcrt, product_bits. */
+int bcrt_geometry(const bcrt_engine* e, uint32_t* out);
+
+/* Plan tables (host copies): primes[P], psi[P], crt_inverse[P] and the CRT
+ * weights[(P+1) * L] (P_j mod q, last row (-P) mod q) -- the values of
+ * RnsPlan in native-smoke/rns-plan.hpp:13-23 and reference.py:50-53. */
+int bcrt_plan_tables(const bcrt_engine* e, uint32_t* primes, uint32_t* psi, uint32_t* crt_inverse,
+                     uint32_t* weights);
+
+/* Host-only plan (no device work): geometry out[10] as bcrt_geometry, and
+ * optionally primes[P], psi[P], crt_inverse[P], weights[(P+1)*L] as
+ * bcrt_plan_tables.  Pass NULL for tables not wanted; call once with only
+ * geometry to learn P.  Restates make_rns_plan (native-smoke/rns-plan.hpp:174-324)
+ * for any q of W bits. */
+int bcrt_plan_host(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, uint32_t* geometry,
+                   uint32_t* primes, uint32_t* psi, uint32_t* crt_inverse, uint32_t* weights,
+                   char* err, size_t errlen);
+
+/* Stage-level access for parity tests and profiling.  Residue planes
+ * ([batch][2][P][N] u32, column-major positions) live in d_planes (NULL = the
+ * engine workspace).  stage_mask bits: 1 encode, 2 forward columns, 4 middle
+ * (forward blocks, pointwise, inverse blocks), 8 inverse columns + CRT scaling,
+ * 16 CRT.  The stages replace, in order, encode_mma32 (rns-encode-mma.cuh:148-252),
+ * forward_first_coalesced (ntt-pipeline.cuh:73-94), fused_middle
+ * (ntt-pipeline.cuh:96-120), inverse_weighted (ntt-weighted.cuh:10-44) and
+ * g1_crt32 (g1-crt.cuh:201-335). */
+int bcrt_run_stages(bcrt_engine* e, uint32_t stage_mask, const uint32_t* d_a, const uint32_t* d_b,
+                    uint32_t* d_c, uint32_t* d_planes, uint32_t batch, void* stream);
+
+/* Words of residue-plane workspace a batch needs (2 * P * N * batch). */
+size_t bcrt_planes_words(const bcrt_engine* e, uint32_t batch);
+
+/* One 32x32x32 int8 MFMA tile on the device: c[i][j] = sum_k a[i][k] * b[j][k]
+ * (row-major int8 32x32 inputs, int32 output); the lane-layout check behind
+ * the encode and CRT GEMMs (replaces native-smoke/mma-smoke.cu). */
+int bcrt_mfma_probe(const int8_t* d_a, const int8_t* d_b, int32_t* d_c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BYTECRT_H */

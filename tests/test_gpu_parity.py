@@ -1,0 +1,190 @@
+"""GPU parity: every stage and the whole product against the oracle, bit-exact.
+
+All calls go through the C ABI (libbytecrt.so via ctypes); torch only holds
+device memory.
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from test_oracle import read_bcrt01
+
+pytestmark = pytest.mark.gpu
+
+GEOMS = [(4096, 100), (8192, 220), (16384, 436), (32768, 868), (65536, 1747)]
+
+
+@pytest.fixture(scope="module")
+def gu():
+    import gpu_util
+    return gpu_util
+
+
+def test_mfma_layout(pkg, gu):
+    rng = np.random.default_rng(1)
+    a = rng.integers(-128, 128, size=(32, 32), dtype=np.int8)
+    b = rng.integers(-128, 128, size=(32, 32), dtype=np.int8)
+    a[0, :] = -128
+    b[0, :] = -128
+    da, db = gu.to_dev(a), gu.to_dev(b)
+    dc = gu.empty_dev((32, 32))
+    pkg.bytecrt.mfma_probe(da, db, dc)
+    got = gu.from_dev(dc, np.int32)
+    want = a.astype(np.int32) @ b.astype(np.int32).T
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("N,W", GEOMS)
+def test_encode_stage(pkg, orc, gu, N, W):
+    q = pkg.datagen.modulus(W, N)
+    e = pkg.Engine(N, W, q)
+    a = pkg.datagen.operands(N, q, 1, edges_first=True)
+    b = pkg.datagen.operands(N, q, 2, edge_fraction=0.1)
+    a[-1] = (1 << 32) - 1  # all-ones limbs: the largest byte pattern
+    planes = gu.empty_dev(e.planes_words(1))
+    e.run_stages(pkg.bytecrt.STAGE_ENCODE, a=gu.to_dev(a), b=gu.to_dev(b), planes=planes)
+    got = gu.from_dev(planes).reshape(2, e.P, N)
+    pl = orc.Plan(N, W, q)
+    for op, x in enumerate((a, b)):
+        want = gu.to_planes(pl.encode(x), e.N1)
+        np.testing.assert_array_equal(got[op], want, err_msg=f"operand {op}")
+
+
+@pytest.mark.parametrize("N,W", [(4096, 100), (32768, 868), (65536, 1747)])
+def test_ntt_stages_dense_protocol(pkg, orc, gu, N, W):
+    """ntt-smoke.cu protocol: dense splitmix inputs with boundary words per prime;
+    the three NTT stages leave t_j = (a*b mod (X^N+1, p_j)) (P/p_j)^-1 mod p_j."""
+    q = pkg.datagen.modulus(W, 3)
+    e = pkg.Engine(N, W, q)
+    pl = orc.Plan(N, W, q)
+    nat = np.zeros((2, e.P, N), dtype=np.uint32)
+    want = np.zeros((e.P, N), dtype=np.uint32)
+    for j, p in enumerate(pl.primes):
+        p = int(p)
+        nat[0, j] = orc.dense_input(N, p, 0)
+        nat[1, j] = orc.dense_input(N, p, 1)
+        c = orc.ntt_negacyclic(p, int(pl.psi[j]), nat[0, j], nat[1, j]).astype(np.uint64)
+        want[j] = (c * int(pl.crt_inv[j]) % p).astype(np.uint32)
+    planes = gu.to_dev(gu.to_planes(nat, e.N1).reshape(-1))
+    e.run_stages(pkg.bytecrt.STAGE_FWD_COLS | pkg.bytecrt.STAGE_MIDDLE | pkg.bytecrt.STAGE_INV_COLS,
+                 planes=planes)
+    got = gu.from_planes(gu.from_dev(planes).reshape(2, e.P, N)[0], e.N1)
+    bad = np.argwhere(got != want)
+    assert bad.size == 0, f"{len(bad)} mismatches, first (prime, coef) = {bad[0].tolist()}"
+
+
+@pytest.mark.parametrize("path", sorted(GOLDEN.glob("g1-*.bin")), ids=lambda p: p.stem)
+def test_crt_stage_bcrt01(pkg, orc, gu, path):
+    fx = read_bcrt01(path)
+    N, W = fx["n"], fx["bits"]
+    q = orc.limbs_to_int(fx["q"])
+    e = pkg.Engine(N, W, q)
+    assert e.P == fx["P"] and e.Kcrt == fx["kpad"]
+    pos = np.zeros((2, e.P, N), dtype=np.uint32)
+    pos[0, :, :fx["m"]] = fx["terms"].T  # plane positions f = 0..15
+    planes = gu.to_dev(pos.reshape(-1))
+    out = gu.empty_dev(N * e.L)
+    e.run_stages(pkg.bytecrt.STAGE_CRT, c=out, planes=planes)
+    got = gu.from_dev(out).reshape(N, e.L)
+    nat = gu.natural_of_position(N, e.N1)
+    np.testing.assert_array_equal(got[nat[:fx["m"]]], fx["golden"])
+    rest = np.ones(N, dtype=bool)
+    rest[nat[:fx["m"]]] = False
+    assert not got[rest].any()
+
+
+def product_case(pkg, orc, gu, N, W, q, seed, edge_fraction=0.0):
+    e = pkg.Engine(N, W, q)
+    a = pkg.datagen.operands(N, q, seed, edges_first=True, edge_fraction=edge_fraction)
+    b = pkg.datagen.operands(N, q, seed + 1, edge_fraction=edge_fraction)
+    out = gu.empty_dev(N * e.L)
+    e.run_device(gu.to_dev(a), gu.to_dev(b), out, 1)
+    got = gu.from_dev(out).reshape(N, e.L)
+    want = orc.Plan(N, W, q).product(a, b)
+    bad = np.argwhere((got != want).any(axis=1))
+    assert bad.size == 0, f"{len(bad)} coefficient mismatches, first {bad[:4].ravel().tolist()}"
+    return a, b, got
+
+
+@pytest.mark.parametrize("N,W", GEOMS)
+def test_product_random_modulus(pkg, orc, gu, N, W):
+    product_case(pkg, orc, gu, N, W, pkg.datagen.modulus(W, 17), seed=5, edge_fraction=0.1)
+
+
+@pytest.mark.parametrize("gap", [1, (1 << 32) - 1])
+def test_product_near_power_modulus(pkg, orc, gu, gap):
+    product_case(pkg, orc, gu, 65536, 1747, (1 << 1747) - gap, seed=9)
+
+
+def test_product_closed_forms(pkg, orc, gu):
+    N, W = 16384, 436
+    q = pkg.datagen.modulus(W, 21)
+    e = pkg.Engine(N, W, q)
+    dg = pkg.datagen
+    b = dg.operands(N, q, 3)
+    bi = orc.rows_to_ints(b)
+
+    def run(a, bb):
+        out = gu.empty_dev(N * e.L)
+        e.run_device(gu.to_dev(a), gu.to_dev(bb), out, 1)
+        return gu.from_dev(out).reshape(N, e.L)
+
+    zero = np.zeros_like(b)
+    np.testing.assert_array_equal(run(zero, b), zero)
+    np.testing.assert_array_equal(run(dg.constant(N, q, 1), b), b)
+    k = 1234
+    mono = np.zeros_like(b)
+    mono[k] = dg.limbs(1, e.L)
+    got = orc.rows_to_ints(run(mono, b))
+    assert got == [(-bi[i - k + N]) % q if i < k else bi[i - k] for i in range(N)]
+    m1 = dg.constant(N, q, q - 1)
+    res = orc.rows_to_ints(run(m1, m1))
+    assert res[0] == 1 and not any(res[1:])
+    # all coefficients q-1 times all q-1: c_k = (k+1) - (N-1-k) = 2k + 2 - N (mod q)
+    f1 = dg.filled(N, q, q - 1)
+    res = orc.rows_to_ints(run(f1, f1))
+    assert res == [(2 * k + 2 - N) % q for k in range(N)]
+
+
+def test_batch_graph_and_whole_call(pkg, orc, gu):
+    """Each product of a batch equals its B=1 result; graph replays equal eager;
+    the host whole-call path equals the device path."""
+    N, W, B = 32768, 868, 4
+    q = pkg.datagen.modulus(W, 33)
+    e = pkg.Engine(N, W, q, max_batch=B)
+    a = np.stack([pkg.datagen.operands(N, q, 40 + i, edge_fraction=0.05) for i in range(B)])
+    b = np.stack([pkg.datagen.operands(N, q, 80 + i) for i in range(B)])
+    da, db = gu.to_dev(a), gu.to_dev(b)
+    out = gu.empty_dev(B * N * e.L)
+    e.run_device(da, db, out, B)
+    batched = gu.from_dev(out).reshape(B, N, e.L)
+    for i in range(B):
+        one = gu.empty_dev(N * e.L)
+        e.run_device(gu.to_dev(a[i]), gu.to_dev(b[i]), one, 1)
+        np.testing.assert_array_equal(gu.from_dev(one).reshape(N, e.L), batched[i])
+    np.testing.assert_array_equal(batched[1], orc.Plan(N, W, q).product(a[1], b[1]))
+    for _ in range(3):
+        out.zero_()
+        e.run_device(da, db, out, B, graph=True)
+        np.testing.assert_array_equal(gu.from_dev(out).reshape(B, N, e.L), batched)
+    whole = e.run(a, b)
+    np.testing.assert_array_equal(whole, batched)
+
+
+def test_fherma_mirror_roundtrip(pkg, orc, gu):
+    bc = pkg.bytecrt
+    N, W, B = 8192, 220, 2
+    q = pkg.datagen.modulus(W, 8)
+    L = (W + 31) // 32
+    st = bc.fherma_init(bc.Point(B=B, N=N, W=W, L=L, q=bc.Tensor(shape=[L], data=bc.limbs_of(q, L))))
+    a = np.stack([pkg.datagen.operands(N, q, i) for i in range(B)])
+    b = np.stack([pkg.datagen.operands(N, q, 10 + i) for i in range(B)])
+    out = bc.fherma_run(st, bc.Inputs(a=bc.Tensor([B, N, L], a.reshape(-1)), b=bc.Tensor([B, N, L], b.reshape(-1))))
+    assert out.c.shape == [B, N, L]
+    c = out.c.data.reshape(B, N, L)
+    pl = orc.Plan(N, W, q)
+    for i in range(B):
+        np.testing.assert_array_equal(c[i], pl.product(a[i], b[i]))
+    with pytest.raises(RuntimeError, match="coefficient count"):
+        bc.fherma_run(st, bc.Inputs(a=bc.Tensor([1], a.reshape(-1)[:5]), b=bc.Tensor([1], b.reshape(-1)[:5])))
+    bc.fherma_free(st)
