@@ -34,7 +34,43 @@ def parse():
     ap.add_argument("--plain-calls", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-stage", choices=["encode", "fwd_cols", "middle", "inv_cols", "crt", "product"],
+                    help="profiling mode: run one stage (or the whole product) --reps times on the batch")
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="profiling mode: device-to-device copy of this many MiB, --reps times")
     return ap.parse_args()
+
+
+STAGE_MASK = {"encode": 1, "fwd_cols": 2, "middle": 4, "inv_cols": 8, "crt": 16, "product": 31}
+
+
+def profile_mode(args, torch):
+    """Work for rocprofv3 runs: nothing but the named kernels after setup."""
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(args.reps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4, "reps": args.reps}), flush=True)
+        return
+    sys.path.insert(0, str(ROOT))
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    q = pkg.datagen.modulus(W, 2026)
+    eng = pkg.Engine(N, W, q, max_batch=B)
+    a = device_operands(torch, B, N, q, 11)
+    b = device_operands(torch, B, N, q, 12)
+    c = torch.empty_like(a)
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()
+    eng.run_stages(31, a=a, b=b, c=c, batch=B, stream=stream)
+    for _ in range(args.reps):
+        eng.run_stages(STAGE_MASK[args.only_stage], a=a, b=b, c=c, batch=B, stream=stream)
+    torch.cuda.synchronize()
+    print(json.dumps({"stage": args.only_stage, "reps": args.reps, "batch": B}), flush=True)
 
 
 def device_operands(torch, batch: int, n: int, q: int, seed: int):
@@ -114,6 +150,9 @@ def main():
         sys.exit(2)
     import torch
 
+    if args.only_stage or args.copy_probe_mib:
+        profile_mode(args, torch)
+        return
     sys.path.insert(0, str(ROOT))
     import __graft_entry__ as ge
     pkg = ge.load_package()
