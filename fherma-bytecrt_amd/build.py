@@ -22,8 +22,10 @@ ARCH = "gfx950"
 SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_engine.cpp", "fherma_solve.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+# -ffp-contract=off: the exact FP64 modular arithmetic relies on every product
+# and sum being rounded where written (explicit fma() calls only).
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-munsafe-fp-atomics"]
+          "-ffp-contract=off"]
 
 
 def _run(cmd):

@@ -132,22 +132,24 @@ int bcrt_create(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, u
         t.Kcrt = g.Kcrt;
         t.Mcrt = g.Mcrt;
         t.qshift = e->plan.qshift;
+        t.qt1 = e->plan.qt1;
         t.inv_qt1 = e->plan.inv_qt1;
         t.primes = upload(*e, e->plan.primes);
-        t.mont_neg_inv = upload(*e, e->plan.mont_neg_inv);
-        t.zf = reinterpret_cast<const uint2*>(upload(*e, e->plan.zf));
-        t.zi = reinterpret_cast<const uint2*>(upload(*e, e->plan.zi));
-        t.scale = reinterpret_cast<const uint2*>(upload(*e, e->plan.scale));
+        t.pd = upload(*e, e->plan.pd);
+        t.zf = upload(*e, e->plan.zf);
+        t.zi = upload(*e, e->plan.zi);
+        t.scale = upload(*e, e->plan.scale);
+        t.enc_prime = upload(*e, e->plan.enc_prime);
         t.wenc = upload(*e, e->plan.wenc);
         t.enc_off = upload(*e, e->plan.enc_off);
-        t.enc_mu = upload(*e, e->plan.enc_mu);
+        t.enc_inv = upload(*e, e->plan.enc_inv);
         t.wcrt = upload(*e, e->plan.wcrt);
         t.corr = upload(*e, e->plan.corr);
         t.invp = upload(*e, e->plan.invp);
         t.q = upload(*e, e->plan.q);
         // The host copies of the big tables are no longer needed.
-        std::vector<uint32_t>().swap(e->plan.zf);
-        std::vector<uint32_t>().swap(e->plan.zi);
+        std::vector<double>().swap(e->plan.zf);
+        std::vector<double>().swap(e->plan.zi);
         void* p = nullptr;
         check(hipMalloc(&p, size_t(max_batch) * 2 * g.P * g.N * sizeof(uint32_t)), "allocate residue planes");
         e->allocations.push_back(p);

@@ -14,8 +14,9 @@
 //
 // The NTT is the merged negacyclic Cooley-Tukey / Gentleman-Sande pair with
 // psi^{+-bitrev(k)} twiddles (no separate twist pass, bit-reversed spectrum
-// consumed in place), Shoup multiplication (v_mul_hi_u32) for twiddles and a
-// Montgomery product for the pointwise step; all values stay canonical in [0, p).
+// consumed in place).  Inside a kernel residues are FP64 integers in a lazy
+// balanced range and every modular product is exact (see mul_mod); between
+// kernels they are canonical u32 in [0, p).
 #include "bytecrt_kernels.hpp"
 
 namespace bcrt {
@@ -28,50 +29,49 @@ constexpr uint32_t kThreads = 256;
 constexpr uint32_t kEncCoefs = 128;  // coefficients per encode workgroup
 constexpr uint32_t kCrtCoefs = 64;   // coefficients per CRT workgroup
 constexpr uint32_t kMidBlocks = 16;  // 256-element blocks per middle workgroup
-constexpr uint32_t kMidStride = 257;
+constexpr uint32_t kMaxPrimesCrt = 128;
 
 // ---------------------------------------------------------------------------
-// Modular arithmetic, p in (2^30, 2^31)
+// Exact modular arithmetic in FP64, p in (2^30, 2^31).
+//
+// mul_mod(x, w): x an integer-valued double with |x| <= 2^40, w in [0, p).
+//   h + l == x * w exactly (TwoProduct by FMA); q = rint(h / p) from one FMA
+//   against 1.5 * 2^52 (|h / p| < 2^51 keeps the sum in the binade where the
+//   spacing is 1); r = (h - q p) + l.  h - q p is an integer of magnitude
+//   below 2^31 + 2^18, so the FMA computes it exactly, and the final sum is
+//   exact too.  Hence r == x w (mod p) exactly, with |r| <= p/2 + 2^-12 p.
+// canon(x): |x| <= 2^40 -> the representative in [0, p).
+// Forward butterflies grow |x| by at most 0.51 p per level, inverse (GS) sums
+// at most double it: eight levels from |x| < p stay below 2^39.
 // ---------------------------------------------------------------------------
-// x * w mod p for any x < 2^32, w < p, ws = floor(w 2^32 / p); the raw
-// remainder is in [0, 2p) and one min() folds it (Shoup 2008 / Harvey 2014).
-__device__ __forceinline__ uint32_t mul_shoup(uint32_t x, uint2 w, uint32_t p) {
-    const uint32_t qh = __umulhi(x, w.y);
-    const uint32_t r = x * w.x - qh * p;
-    return min(r, r - p);
+constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
+
+__device__ __forceinline__ double mul_mod(double x, double w, double p, double pinv) {
+    const double h = x * w;
+    const double l = fma(x, w, -h);
+    const double q = fma(h, pinv, kRound) - kRound;
+    return fma(-q, p, h) + l;
 }
 
-__device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b, uint32_t p) {
-    const uint32_t s = a + b;
-    return min(s, s - p);
-}
-
-__device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) {
-    const uint32_t d = a - b + p;
-    return min(d, d - p);
-}
-
-// a * b * 2^-32 mod p (Montgomery 1985), a, b < p; ninv = -p^-1 mod 2^32.
-__device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t ninv) {
-    const uint64_t t = uint64_t(a) * b;
-    const uint32_t m = uint32_t(t) * ninv;
-    const uint32_t r = uint32_t((t + uint64_t(m) * p) >> 32);
-    return min(r, r - p);
+__device__ __forceinline__ uint32_t canon(double x, double p, double pinv) {
+    const double q = fma(x, pinv, kRound) - kRound;
+    double r = fma(-q, p, x);  // exact, |r| <= p/2 + 1
+    r = r < 0.0 ? r + p : r;
+    return uint32_t(r);
 }
 
 // (x, y) -> (x + w y, x - w y)
-__device__ __forceinline__ void ct_bfly(uint32_t& x, uint32_t& y, uint2 w, uint32_t p) {
-    const uint32_t v = mul_shoup(y, w, p);
-    const uint32_t s = add_mod(x, v, p);
-    y = sub_mod(x, v, p);
-    x = s;
+__device__ __forceinline__ void ct_bfly(double& x, double& y, double w, double p, double pinv) {
+    const double v = mul_mod(y, w, p, pinv);
+    y = x - v;
+    x = x + v;
 }
 
 // (x, y) -> (x + y, (x - y) w)
-__device__ __forceinline__ void gs_bfly(uint32_t& x, uint32_t& y, uint2 w, uint32_t p) {
-    const uint32_t s = add_mod(x, y, p);
-    y = mul_shoup(x - y + p, w, p);
-    x = s;
+__device__ __forceinline__ void gs_bfly(double& x, double& y, double w, double p, double pinv) {
+    const double d = x - y;
+    x = x + y;
+    y = mul_mod(d, w, p, pinv);
 }
 
 // K transform levels on the thread's 16 registers, which hold (16 >> K)
@@ -79,22 +79,20 @@ __device__ __forceinline__ void gs_bfly(uint32_t& x, uint32_t& y, uint2 w, uint3
 // a length-M transform; the sub-block's outer index is o0 + sb; the twiddle
 // of group g is z[(mult << (e0 + l)) + ((o0 + sb) << l) + g].
 template <int K>
-__device__ __forceinline__ void ct_round(uint32_t (&v)[16], const uint2* __restrict__ z,
-                                         uint32_t mult, uint32_t e0, uint32_t o0, uint32_t p) {
+__device__ __forceinline__ void ct_round(double (&v)[16], const double* __restrict__ z, uint32_t mult,
+                                         uint32_t e0, uint32_t o0, double p, double pinv) {
 #pragma unroll
     for (int l = 0; l < K; ++l) {
-        constexpr int dummy = 0;
-        (void)dummy;
         const int half = 1 << (K - 1 - l);
 #pragma unroll
         for (int sb = 0; sb < (16 >> K); ++sb) {
 #pragma unroll
             for (int g = 0; g < (1 << l); ++g) {
-                const uint2 w = z[(mult << (e0 + l)) + ((o0 + sb) << l) + g];
+                const double w = z[(mult << (e0 + l)) + ((o0 + sb) << l) + g];
 #pragma unroll
                 for (int x = 0; x < half; ++x) {
                     const int i0 = (sb << K) + g * 2 * half + x;
-                    ct_bfly(v[i0], v[i0 + half], w, p);
+                    ct_bfly(v[i0], v[i0 + half], w, p, pinv);
                 }
             }
         }
@@ -102,8 +100,8 @@ __device__ __forceinline__ void ct_round(uint32_t (&v)[16], const uint2* __restr
 }
 
 template <int K>
-__device__ __forceinline__ void gs_round(uint32_t (&v)[16], const uint2* __restrict__ z,
-                                         uint32_t mult, uint32_t e0, uint32_t o0, uint32_t p) {
+__device__ __forceinline__ void gs_round(double (&v)[16], const double* __restrict__ z, uint32_t mult,
+                                         uint32_t e0, uint32_t o0, double p, double pinv) {
 #pragma unroll
     for (int l = K - 1; l >= 0; --l) {
         const int half = 1 << (K - 1 - l);
@@ -111,16 +109,23 @@ __device__ __forceinline__ void gs_round(uint32_t (&v)[16], const uint2* __restr
         for (int sb = 0; sb < (16 >> K); ++sb) {
 #pragma unroll
             for (int g = 0; g < (1 << l); ++g) {
-                const uint2 w = z[(mult << (e0 + l)) + ((o0 + sb) << l) + g];
+                const double w = z[(mult << (e0 + l)) + ((o0 + sb) << l) + g];
 #pragma unroll
                 for (int x = 0; x < half; ++x) {
                     const int i0 = (sb << K) + g * 2 * half + x;
-                    gs_bfly(v[i0], v[i0 + half], w, p);
+                    gs_bfly(v[i0], v[i0 + half], w, p, pinv);
                 }
             }
         }
     }
 }
+
+struct EncPrime {
+    double off;       // offset making the biased dot sum non-negative (< 2^47, exact)
+    double inv;       // a double <= 1/p
+    uint32_t p;
+    uint32_t off_lo;  // off mod 2^32
+};
 
 __device__ __forceinline__ uint32_t* plane_ptr(const DevTables& t, uint32_t* planes, uint32_t prod,
                                                uint32_t op, uint32_t j) {
@@ -138,16 +143,34 @@ __global__ __launch_bounds__(kThreads) void k_encode(DevTables t, const uint32_t
                                                      const uint32_t* __restrict__ b,
                                                      uint32_t* __restrict__ planes) {
     extern __shared__ uint32_t enc_tile[];
+    // Per-prime reduction constants {off, 1/p rounded down, p, off mod 2^32}.
+    __shared__ EncPrime cst[kMaxPrimesCrt];
+    for (uint32_t j = threadIdx.x; j < t.P; j += kThreads)
+        cst[j] = reinterpret_cast<const EncPrime*>(t.enc_prime)[j];
     const uint32_t op = blockIdx.y, prod = blockIdx.z;
     const uint32_t* in = (op ? b : a) + size_t(prod) * t.N * t.L;
     const uint32_t f0 = blockIdx.x * kEncCoefs;
     const uint32_t row_words = t.Kenc / 4;
     const uint32_t stride_words = row_words + 4;
-    for (uint32_t i = threadIdx.x; i < kEncCoefs * row_words; i += kThreads) {
-        const uint32_t c = i / row_words, w = i - c * row_words;
-        const uint32_t f = f0 + c;
-        const uint32_t n = (f >> t.n1) + t.N2 * (f & (t.N1 - 1));
-        enc_tile[c * stride_words + w] = w < t.L ? in[size_t(n) * t.L + w] : 0u;
+    {
+        // Each wave loads one coefficient row per step (lanes over limbs); all
+        // 32 loads of a thread are issued before the first LDS store.
+        constexpr uint32_t kRowsPerStep = kThreads / 64;
+        constexpr uint32_t kSteps = kEncCoefs / kRowsPerStep;
+        const uint32_t w = threadIdx.x & 63, c0 = threadIdx.x >> 6;
+        uint32_t regs[kSteps];
+#pragma unroll
+        for (uint32_t it = 0; it < kSteps; ++it) {
+            const uint32_t f = f0 + c0 + kRowsPerStep * it;
+            const uint32_t n = (f >> t.n1) + t.N2 * (f & (t.N1 - 1));
+            regs[it] = w < t.L ? in[size_t(n) * t.L + w] : 0u;
+        }
+        // Bytes are stored biased (x ^ 0x80 = x - 128) for the signed MFMA.
+        if (w < row_words) {
+#pragma unroll
+            for (uint32_t it = 0; it < kSteps; ++it)
+                enc_tile[(c0 + kRowsPerStep * it) * stride_words + w] = regs[it] ^ 0x80808080u;
+        }
     }
     __syncthreads();
 
@@ -155,38 +178,41 @@ __global__ __launch_bounds__(kThreads) void k_encode(DevTables t, const uint32_t
     const uint32_t row = lane & 31, kh = lane >> 5;
     const uint8_t* tile_bytes = reinterpret_cast<const uint8_t*>(enc_tile);
     const uint32_t stride_bytes = stride_words * 4;
-    const v4i bias = {int(0x80808080u), int(0x80808080u), int(0x80808080u), int(0x80808080u)};
+    constexpr uint32_t kMaxKSteps = 8;  // Kenc <= 256
+    const uint32_t ksteps = t.Kenc / 32;
     for (uint32_t pt = wave; pt < t.PTenc; pt += 4) {
-        v16i acc[4];
-#pragma unroll
-        for (int ct = 0; ct < 4; ++ct) acc[ct] = v16i{};
+        // The tile's weight fragments for every k step, loaded once.
         const int8_t* wrow = t.wenc + (size_t(pt) * 32 + row) * t.Kenc + 16 * kh;
-        for (uint32_t kk = 0; kk < t.Kenc; kk += 32) {
-            const v4i A = *reinterpret_cast<const v4i*>(wrow + kk);
+        v4i A[kMaxKSteps];
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                const v4i B = *reinterpret_cast<const v4i*>(tile_bytes + (ct * 32 + row) * stride_bytes +
-                                                            kk + 16 * kh) ^ bias;
-                acc[ct] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, B, acc[ct], 0, 0, 0);
+        for (uint32_t s2 = 0; s2 < kMaxKSteps; ++s2)
+            A[s2] = s2 < ksteps ? *reinterpret_cast<const v4i*>(wrow + 32 * s2) : v4i{};
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            v16i acc = v16i{};
+            const uint8_t* brow = tile_bytes + (ct * 32 + row) * stride_bytes + 16 * kh;
+#pragma unroll
+            for (uint32_t s2 = 0; s2 < kMaxKSteps; ++s2) {
+                if (s2 < ksteps) {
+                    const v4i B = *reinterpret_cast<const v4i*>(brow + 32 * s2);
+                    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[s2], B, acc, 0, 0, 0);
+                }
             }
-        }
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const uint32_t j = pt * 8 + 2 * g + kh;
-            if (j < t.P) {
-                const uint32_t p = t.primes[j];
-                const uint64_t off = t.enc_off[j], mu = t.enc_mu[j];
-                uint32_t* dst = plane_ptr(t, planes, prod, op, j) + f0 + row;
-#pragma unroll
-                for (int ct = 0; ct < 4; ++ct) {
-                    const int64_t v = int64_t(acc[ct][4 * g]) + (int64_t(acc[ct][4 * g + 1]) << 8) +
-                                      (int64_t(acc[ct][4 * g + 2]) << 16) +
-                                      (int64_t(acc[ct][4 * g + 3]) << 24);
-                    const uint64_t u = uint64_t(v + int64_t(off));
-                    // u < 2^47: qh = floor(u mu / 2^64) is floor(u/p) or one less.
-                    const uint64_t qh = __umul64hi(u, mu);
-                    const uint32_t r = uint32_t(u) - uint32_t(qh) * p;
-                    dst[ct * 32] = min(r, r - p);
+            for (int g = 0; g < 4; ++g) {
+                const uint32_t j = pt * 8 + 2 * g + kh;
+                if (j < t.P) {
+                    const EncPrime c = cst[j];
+                    // digit sums |C| < 2^22: the two byte pairs fit int32
+                    const int32_t lo = acc[4 * g] + acc[4 * g + 1] * 256;
+                    const int32_t hi = acc[4 * g + 2] + acc[4 * g + 3] * 256;
+                    // u = lo + 2^16 hi + off, 0 <= u < 2^47: exact in FP64; c.inv <= 1/p
+                    // makes qd = floor(u/p) or one less, so r = u - qd p is in [0, 2p).
+                    const double ud = fma(double(hi), 65536.0, double(lo)) + c.off;
+                    const uint32_t qd = uint32_t(ud * c.inv);
+                    const uint32_t u32 = uint32_t(lo) + (uint32_t(hi) << 16) + c.off_lo;
+                    const uint32_t r = u32 - qd * c.p;
+                    plane_ptr(t, planes, prod, op, j)[f0 + ct * 32 + row] = min(r, r - c.p);
                 }
             }
         }
@@ -194,167 +220,246 @@ __global__ __launch_bounds__(kThreads) void k_encode(DevTables t, const uint32_t
 }
 
 // ---------------------------------------------------------------------------
+// LDS exchange layout shared by the NTT kernels: element u of a length-M
+// sequence c sits at c * S + u + (u >> 4).  With S >= M + M/16 and
+// S == T (mod 32), where T = M/16 threads own one sequence, both exchange
+// patterns (thread tau holding u = tau + T x, or u = 16 tau + x) are
+// bank-conflict free for 32 banks per half-wave and 64 banks per wave.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t skew(uint32_t u) { return u + (u >> 4); }
+
+template <uint32_t M>
+constexpr uint32_t exchange_stride() {
+    constexpr uint32_t T = M / 16, base = M + M / 16;
+    return base + (T + 32 - base % 32) % 32;
+}
+
+// ---------------------------------------------------------------------------
 // Column transforms (first / last n1 levels).  T = N1/16 threads per column,
-// 16 registers each; 4096/N1 columns per workgroup, all of one prime.
+// 16 registers each; 4096/N1 columns per workgroup, all of one prime.  The
+// exchange between the two register rounds goes through LDS as FP64.
 // ---------------------------------------------------------------------------
 template <int NLOG1>
 __global__ __launch_bounds__(kThreads) void k_fwd_cols(DevTables t, uint32_t* __restrict__ planes) {
-    constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = kThreads / T, S = N1 + 1;
+    constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = kThreads / T, S = exchange_stride<N1>();
     constexpr int K2 = NLOG1 - 4;
-    __shared__ uint32_t s[K2 > 0 ? CPW * S : 1];
+    __shared__ double s[K2 > 0 ? CPW * S : 1];
     const uint32_t wg_per_prime = t.N2 / CPW;
     const uint32_t j = blockIdx.x / wg_per_prime;
     const uint32_t r0 = (blockIdx.x % wg_per_prime) * CPW;
     const uint32_t cw = threadIdx.x / T, tau = threadIdx.x % T;
     uint32_t* col = plane_ptr(t, planes, blockIdx.z, blockIdx.y, j) + size_t(r0 + cw) * N1;
-    const uint2* z = t.zf + size_t(j) * t.N;
-    const uint32_t p = t.primes[j];
-    uint32_t v[16];
+    const double* z = t.zf + size_t(j) * t.N;
+    const double p = t.pd[2 * j], pinv = t.pd[2 * j + 1];
+    double v[16];
 #pragma unroll
-    for (int x = 0; x < 16; ++x) v[x] = col[tau + T * x];
-    ct_round<4>(v, z, 1, 0, 0, p);
+    for (int x = 0; x < 16; ++x) v[x] = double(col[tau + T * x]);
+    ct_round<4>(v, z, 1, 0, 0, p, pinv);
     if constexpr (K2 > 0) {
+        double* sc = s + cw * S;
 #pragma unroll
-        for (int x = 0; x < 16; ++x) s[cw * S + tau + T * x] = v[x];
+        for (int x = 0; x < 16; ++x) sc[skew(tau + T * x)] = v[x];
         __syncthreads();
 #pragma unroll
-        for (int x = 0; x < 16; ++x) v[x] = s[cw * S + 16 * tau + x];
-        ct_round<K2>(v, z, 1, 4, tau * (16 >> K2), p);
+        for (int x = 0; x < 16; ++x) v[x] = sc[skew(16 * tau + x)];
+        ct_round<K2>(v, z, 1, 4, tau * (16 >> K2), p, pinv);
 #pragma unroll
-        for (int x = 0; x < 16; ++x) col[16 * tau + x] = v[x];
+        for (int x = 0; x < 16; x += 4)
+            *reinterpret_cast<uint4*>(col + 16 * tau + x) =
+                make_uint4(canon(v[x], p, pinv), canon(v[x + 1], p, pinv), canon(v[x + 2], p, pinv),
+                           canon(v[x + 3], p, pinv));
     } else {
 #pragma unroll
-        for (int x = 0; x < 16; ++x) col[x] = v[x];
+        for (int x = 0; x < 16; ++x) col[x] = canon(v[x], p, pinv);
     }
 }
 
 template <int NLOG1>
 __global__ __launch_bounds__(kThreads) void k_inv_cols(DevTables t, uint32_t* __restrict__ planes) {
-    constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = kThreads / T, S = N1 + 1;
+    constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = kThreads / T, S = exchange_stride<N1>();
     constexpr int K2 = NLOG1 - 4;
-    __shared__ uint32_t s[K2 > 0 ? CPW * S : 1];
+    __shared__ double s[K2 > 0 ? CPW * S : 1];
     const uint32_t wg_per_prime = t.N2 / CPW;
     const uint32_t j = blockIdx.x / wg_per_prime;
     const uint32_t r0 = (blockIdx.x % wg_per_prime) * CPW;
     const uint32_t cw = threadIdx.x / T, tau = threadIdx.x % T;
     uint32_t* col = plane_ptr(t, planes, blockIdx.z, 0, j) + size_t(r0 + cw) * N1;
-    const uint2* z = t.zi + size_t(j) * t.N;
-    const uint32_t p = t.primes[j];
-    uint32_t v[16];
+    const double* z = t.zi + size_t(j) * t.N;
+    const double p = t.pd[2 * j], pinv = t.pd[2 * j + 1];
+    double v[16];
     if constexpr (K2 > 0) {
 #pragma unroll
-        for (int x = 0; x < 16; ++x) v[x] = col[16 * tau + x];
-        gs_round<K2>(v, z, 1, 4, tau * (16 >> K2), p);
+        for (int x = 0; x < 16; x += 4) {
+            const uint4 q4 = *reinterpret_cast<const uint4*>(col + 16 * tau + x);
+            v[x] = double(q4.x);
+            v[x + 1] = double(q4.y);
+            v[x + 2] = double(q4.z);
+            v[x + 3] = double(q4.w);
+        }
+        gs_round<K2>(v, z, 1, 4, tau * (16 >> K2), p, pinv);
+        double* sc = s + cw * S;
 #pragma unroll
-        for (int x = 0; x < 16; ++x) s[cw * S + 16 * tau + x] = v[x];
+        for (int x = 0; x < 16; ++x) sc[skew(16 * tau + x)] = v[x];
         __syncthreads();
 #pragma unroll
-        for (int x = 0; x < 16; ++x) v[x] = s[cw * S + tau + T * x];
+        for (int x = 0; x < 16; ++x) v[x] = sc[skew(tau + T * x)];
     } else {
 #pragma unroll
-        for (int x = 0; x < 16; ++x) v[x] = col[x];
+        for (int x = 0; x < 16; ++x) v[x] = double(col[x]);
     }
-    gs_round<4>(v, z, 1, 0, 0, p);
-    const uint2 sc = t.scale[j];
+    gs_round<4>(v, z, 1, 0, 0, p, pinv);
+    const double sc2 = t.scale[j];
 #pragma unroll
-    for (int x = 0; x < 16; ++x) col[tau + T * x] = mul_shoup(v[x], sc, p);
+    for (int x = 0; x < 16; ++x) col[tau + T * x] = canon(mul_mod(v[x], sc2, p, pinv), p, pinv);
 }
 
 // ---------------------------------------------------------------------------
 // Middle: 16 blocks (256 contiguous natural coefficients each) of one prime,
 // forward levels n1..n-1 for A and B, pointwise product, inverse levels back.
+// Grid (N1/16, batch, P): workgroups of one prime run together so its block
+// twiddles stay in L2 across the whole batch.  One FP64 exchange buffer is
+// reused for A, then B, then the product.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kMidS = exchange_stride<256>();
+
+// Thread u owns row u of the tile: 16 consecutive blocks (64 contiguous bytes).
+__device__ __forceinline__ void mid_fetch(uint4 (&q4)[4], const uint32_t* __restrict__ plane, uint32_t N1,
+                                          uint32_t b0) {
+    const uint4* src = reinterpret_cast<const uint4*>(plane + size_t(threadIdx.x) * N1 + b0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q4[k] = src[k];
+}
+
+__device__ __forceinline__ void mid_store_lds(double* __restrict__ dst, const uint4 (&q4)[4]) {
+    const uint32_t o = skew(threadIdx.x);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        dst[(4 * k + 0) * kMidS + o] = double(q4[k].x);
+        dst[(4 * k + 1) * kMidS + o] = double(q4[k].y);
+        dst[(4 * k + 2) * kMidS + o] = double(q4[k].z);
+        dst[(4 * k + 3) * kMidS + o] = double(q4[k].w);
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __restrict__ planes) {
-    __shared__ uint32_t sa[kMidBlocks * kMidStride];
-    __shared__ uint32_t sb[kMidBlocks * kMidStride];
-    const uint32_t groups = t.N1 / kMidBlocks;
-    const uint32_t j = blockIdx.x / groups;
-    const uint32_t b0 = (blockIdx.x % groups) * kMidBlocks;
-    const uint32_t prod = blockIdx.z;
-    const uint32_t p = t.primes[j];
+    __shared__ double sx[kMidBlocks * kMidS];
+    const uint32_t j = blockIdx.z;
+    const uint32_t b0 = blockIdx.x * kMidBlocks;
+    const uint32_t prod = blockIdx.y;
+    const double p = t.pd[2 * j], pinv = t.pd[2 * j + 1];
     uint32_t* pa = plane_ptr(t, planes, prod, 0, j);
     uint32_t* pb = plane_ptr(t, planes, prod, 1, j);
-    for (uint32_t i = threadIdx.x; i < kMidBlocks * 256; i += kThreads) {
-        const uint32_t u = i >> 4, bl = i & 15;
-        sa[bl * kMidStride + u] = pa[size_t(u) * t.N1 + b0 + bl];
-        sb[bl * kMidStride + u] = pb[size_t(u) * t.N1 + b0 + bl];
-    }
-    __syncthreads();
     const uint32_t bl = threadIdx.x >> 4, tau = threadIdx.x & 15;
     const uint32_t mult = t.N1 + b0 + bl;
-    const uint2* zf = t.zf + size_t(j) * t.N;
-    const uint2* zi = t.zi + size_t(j) * t.N;
-    uint32_t* ra = sa + bl * kMidStride;
-    uint32_t* rb = sb + bl * kMidStride;
-    uint32_t v[16], w[16];
-#pragma unroll
-    for (int x = 0; x < 16; ++x) {
-        v[x] = ra[tau + 16 * x];
-        w[x] = rb[tau + 16 * x];
-    }
-    ct_round<4>(v, zf, mult, 0, 0, p);
-    ct_round<4>(w, zf, mult, 0, 0, p);
-#pragma unroll
-    for (int x = 0; x < 16; ++x) {
-        ra[tau + 16 * x] = v[x];
-        rb[tau + 16 * x] = w[x];
-    }
+    const double* zf = t.zf + size_t(j) * t.N;
+    const double* zi = t.zi + size_t(j) * t.N;
+    double* rx = sx + bl * kMidS;
+    double v[16], w[16];
+    // Both operands' tiles are fetched up front; B waits in registers while A
+    // goes through forward sub-levels 0..3 (positions tau + 16x), then 4..7 (16 tau + x).
+    uint4 qa[4], qb[4];
+    mid_fetch(qa, pa, t.N1, b0);
+    mid_fetch(qb, pb, t.N1, b0);
+    mid_store_lds(sx, qa);
     __syncthreads();
 #pragma unroll
-    for (int x = 0; x < 16; ++x) {
-        v[x] = ra[16 * tau + x];
-        w[x] = rb[16 * tau + x];
-    }
-    ct_round<4>(v, zf, mult, 4, tau, p);
-    ct_round<4>(w, zf, mult, 4, tau, p);
-    const uint32_t ninv = t.mont_neg_inv[j];
+    for (int x = 0; x < 16; ++x) v[x] = rx[skew(tau + 16 * x)];
+    ct_round<4>(v, zf, mult, 0, 0, p, pinv);
 #pragma unroll
-    for (int x = 0; x < 16; ++x) v[x] = mont_mul(v[x], w[x], p, ninv);
-    gs_round<4>(v, zi, mult, 4, tau, p);
-#pragma unroll
-    for (int x = 0; x < 16; ++x) ra[16 * tau + x] = v[x];
+    for (int x = 0; x < 16; ++x) rx[skew(tau + 16 * x)] = v[x];
     __syncthreads();
 #pragma unroll
-    for (int x = 0; x < 16; ++x) v[x] = ra[tau + 16 * x];
-    gs_round<4>(v, zi, mult, 0, 0, p);
-#pragma unroll
-    for (int x = 0; x < 16; ++x) ra[tau + 16 * x] = v[x];
+    for (int x = 0; x < 16; ++x) v[x] = rx[skew(16 * tau + x)];
+    ct_round<4>(v, zf, mult, 4, tau, p, pinv);
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kMidBlocks * 256; i += kThreads) {
-        const uint32_t u = i >> 4, b = i & 15;
-        pa[size_t(u) * t.N1 + b0 + b] = sa[b * kMidStride + u];
+    // operand B
+    mid_store_lds(sx, qb);
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 16; ++x) w[x] = rx[skew(tau + 16 * x)];
+    ct_round<4>(w, zf, mult, 0, 0, p, pinv);
+#pragma unroll
+    for (int x = 0; x < 16; ++x) rx[skew(tau + 16 * x)] = w[x];
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 16; ++x) w[x] = rx[skew(16 * tau + x)];
+    ct_round<4>(w, zf, mult, 4, tau, p, pinv);
+    // pointwise product (|v|, |w| < 2^34) and inverse sub-levels 7..4
+#pragma unroll
+    for (int x = 0; x < 16; ++x) v[x] = mul_mod(v[x], w[x], p, pinv);
+    gs_round<4>(v, zi, mult, 4, tau, p, pinv);
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 16; ++x) rx[skew(16 * tau + x)] = v[x];
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 16; ++x) v[x] = rx[skew(tau + 16 * x)];
+    gs_round<4>(v, zi, mult, 0, 0, p, pinv);
+    __syncthreads();
+    {
+        // canonical residues back in place, then row u out as four 16-byte stores
+        uint32_t* sy = reinterpret_cast<uint32_t*>(sx);
+#pragma unroll
+        for (int x = 0; x < 16; ++x) sy[bl * kMidS + skew(tau + 16 * x)] = canon(v[x], p, pinv);
+        __syncthreads();
+        const uint32_t u = threadIdx.x;
+        uint4* dst = reinterpret_cast<uint4*>(pa + size_t(u) * t.N1 + b0);
+        const uint32_t o = skew(u);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            dst[k] = make_uint4(sy[(4 * k) * kMidS + o], sy[(4 * k + 1) * kMidS + o],
+                                sy[(4 * k + 2) * kMidS + o], sy[(4 * k + 3) * kMidS + o]);
     }
 }
 
 // ---------------------------------------------------------------------------
-// CRT: 64 coefficients (consecutive column-major positions).  Terms t_j are
-// in operand-A planes.  alpha = round(sum t_j / p_j) in FP64 (the exact value
-// is within 1/4 of an integer; the FP64 error is below 2^-37).  Then
+// CRT: 64 coefficients (consecutive column-major positions), four waves.
+// Terms t_j are in operand-A planes.  alpha = round(sum t_j / p_j) in FP64 (the
+// exact value is within 1/4 of an integer; the FP64 error is below 2^-37).  Then
 //   z = sum_{j<=P, a<4} byte_a(t_j) * W[j,a],  W[j,a] = 256^a w_j mod q,
-// as an int8 MFMA GEMM over balanced signed weight digits and biased bytes,
-// a signed carry chain over 4-digit limbs, and r = z - Q q with Q from the
-// top 64 bits (three conditional subtractions at most).
+// as an int8 MFMA GEMM over balanced signed weight digits and biased bytes
+// (wave w computes digit-row tiles w, w+4, ...), a signed carry chain per
+// coefficient, and r = z - Q q with Q = floor(z_hi / (q_hi + 1)) (exact),
+// which is floor(z/q) or one less, so one conditional subtraction finishes.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
-                                                  uint32_t* __restrict__ out) {
+constexpr uint32_t kCrtThreads = 256;
+constexpr uint32_t kCrtMaxK = (4 * (kMaxPrimesCrt + 1) + 31) / 32;  // k steps of 32
+
+__global__ __launch_bounds__(kCrtThreads) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
+                                                     uint32_t* __restrict__ out) {
     extern __shared__ __align__(16) uint8_t crt_smem[];
-    const uint32_t tstride = t.Kcrt + 16;  // bytes per term row
+    const uint32_t L = t.L, P = t.P;
+    const uint32_t tstride = t.Kcrt + 16;  // bytes per term row (odd multiple of 16: conflict-free b128)
     const uint32_t tw = tstride / 4;
-    const uint32_t kw = t.Kcrt / 4;
+    const uint32_t groups = t.Kcrt / 16;   // 4-term groups per row
+    const uint32_t DS = L | 1;             // odd digit-row stride
     uint8_t* tmat = crt_smem;
     uint32_t* tmw = reinterpret_cast<uint32_t*>(crt_smem);
-    int64_t* lo = reinterpret_cast<int64_t*>(crt_smem + kCrtCoefs * tstride);
-    const uint32_t L = t.L, P = t.P, L1 = L + 1;
+    uint32_t* dig = tmw;  // [64][DS], reuses the term rows once the GEMM is done
+    int64_t* lo = reinterpret_cast<int64_t*>(crt_smem + kCrtCoefs * tstride);  // [L+1][64]
     const uint32_t f0 = blockIdx.x * kCrtCoefs, prod = blockIdx.z;
-    const uint32_t* pa = planes + size_t(prod) * 2 * P * t.N;
-    for (uint32_t i = threadIdx.x; i < P * kCrtCoefs; i += kThreads) {
-        const uint32_t j = i >> 6, c = i & 63;
-        tmw[c * tw + j] = pa[size_t(j) * t.N + f0 + c];
-    }
-    const uint32_t padw = kw - P;
-    for (uint32_t i = threadIdx.x; i < kCrtCoefs * padw; i += kThreads) {
-        const uint32_t c = i / padw, w = P + (i - c * padw);
-        tmw[c * tw + w] = 0u;
+    const uint32_t* pa = planes + size_t(prod) * 2 * P * t.N + f0;
+    {
+        // lane c gathers four consecutive terms of its coefficient (coalesced per
+        // plane); every load of a thread is issued before its LDS stores.
+        constexpr uint32_t kIters = (kCrtMaxK * 2 + 3) / 4;
+        const uint32_t c = threadIdx.x & 63, g0 = threadIdx.x >> 6;
+        uint32_t v[kIters][4];
+#pragma unroll
+        for (uint32_t it = 0; it < kIters; ++it) {
+            const uint32_t g = g0 + 4 * it;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t j = 4 * g + k;
+                v[it][k] = (g < groups && j < P) ? pa[size_t(j) * t.N + c] : 0u;
+            }
+        }
+#pragma unroll
+        for (uint32_t it = 0; it < kIters; ++it) {
+            const uint32_t g = g0 + 4 * it;
+            if (g < groups)
+                *reinterpret_cast<uint4*>(tmw + c * tw + 4 * g) = make_uint4(v[it][0], v[it][1], v[it][2], v[it][3]);
+        }
     }
     __syncthreads();
     {
@@ -370,89 +475,98 @@ __global__ __launch_bounds__(kThreads) void k_crt(DevTables t, const uint32_t* _
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t row = lane & 31, kh = lane >> 5;
     const v4i bias = {int(0x80808080u), int(0x80808080u), int(0x80808080u), int(0x80808080u)};
+    const uint32_t ksteps = t.Kcrt / 32;
     for (uint32_t mt = wave; mt < t.Mcrt / 32; mt += 4) {
-        v16i acc0 = v16i{}, acc1 = v16i{};
         const int8_t* wrow = t.wcrt + size_t(mt * 32 + row) * t.Kcrt + 16 * kh;
-        for (uint32_t kk = 0; kk < t.Kcrt; kk += 32) {
-            const v4i A = *reinterpret_cast<const v4i*>(wrow + kk);
-            const v4i B0 = *reinterpret_cast<const v4i*>(tmat + row * tstride + kk + 16 * kh) ^ bias;
-            const v4i B1 = *reinterpret_cast<const v4i*>(tmat + (32 + row) * tstride + kk + 16 * kh) ^ bias;
-            acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, B0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, B1, acc1, 0, 0, 0);
+        v4i A[kCrtMaxK];
+#pragma unroll
+        for (uint32_t k = 0; k < kCrtMaxK; ++k)
+            A[k] = k < ksteps ? *reinterpret_cast<const v4i*>(wrow + 32 * k) : v4i{};
+        v16i acc0 = v16i{}, acc1 = v16i{};
+#pragma unroll
+        for (uint32_t k = 0; k < kCrtMaxK; ++k) {
+            if (k < ksteps) {
+                const v4i B0 = *reinterpret_cast<const v4i*>(tmat + row * tstride + 32 * k + 16 * kh) ^ bias;
+                const v4i B1 = *reinterpret_cast<const v4i*>(tmat + (32 + row) * tstride + 32 * k + 16 * kh) ^ bias;
+                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B1, acc1, 0, 0, 0);
+            }
         }
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const uint32_t limb = mt * 8 + 2 * g + kh;
             if (limb <= L) {
                 const int64_t cr = t.corr[limb];
-                lo[row * L1 + limb] = int64_t(acc0[4 * g]) + (int64_t(acc0[4 * g + 1]) << 8) +
-                                      (int64_t(acc0[4 * g + 2]) << 16) +
-                                      (int64_t(acc0[4 * g + 3]) << 24) + cr;
-                lo[(32 + row) * L1 + limb] = int64_t(acc1[4 * g]) + (int64_t(acc1[4 * g + 1]) << 8) +
-                                             (int64_t(acc1[4 * g + 2]) << 16) +
-                                             (int64_t(acc1[4 * g + 3]) << 24) + cr;
+                lo[limb * kCrtCoefs + row] = int64_t(acc0[4 * g]) + int64_t(acc0[4 * g + 1]) * 256 +
+                                             int64_t(acc0[4 * g + 2]) * 65536 +
+                                             int64_t(acc0[4 * g + 3]) * 16777216 + cr;
+                lo[limb * kCrtCoefs + 32 + row] = int64_t(acc1[4 * g]) + int64_t(acc1[4 * g + 1]) * 256 +
+                                                  int64_t(acc1[4 * g + 2]) * 65536 +
+                                                  int64_t(acc1[4 * g + 3]) * 16777216 + cr;
             }
         }
     }
     __syncthreads();
 
     if (threadIdx.x < kCrtCoefs) {
-        int64_t* z = lo + threadIdx.x * L1;
+        const uint32_t c = threadIdx.x;
+        uint32_t* d = dig + c * DS;
         int64_t carry = 0;
         for (uint32_t i = 0; i < L; ++i) {
-            const int64_t tot = carry + z[i];
-            z[i] = int64_t(uint32_t(tot));
+            const int64_t tot = carry + lo[i * kCrtCoefs + c];
+            d[i] = uint32_t(tot);
             carry = tot >> 32;
         }
-        int64_t top = carry + z[L];  // z = sum digits 2^(32 i) + top 2^(32 L), 0 <= z < 2^(W+17)
+        // z = sum d_i 2^(32 i) + top 2^(32 L), 0 <= z < 2^(W+18)
+        int64_t top = carry + lo[L * kCrtCoefs + c];
         auto word = [&](uint32_t i) -> uint64_t {
-            return i < L ? uint64_t(uint32_t(z[i])) : (i == L ? uint64_t(top) : 0);
+            return i < L ? uint64_t(d[i]) : (i == L ? uint64_t(top) : 0);
         };
         const uint32_t a = t.qshift >> 5, bsh = t.qshift & 31;
-        uint64_t zt = (word(a) | (word(a + 1) << 32)) >> bsh;
+        uint64_t zt = (word(a) | (word(a + 1) << 32)) >> bsh;  // z >> (W-32) < 2^50
         if (bsh) zt |= word(a + 2) << (64 - bsh);
-        const uint64_t qf = uint64_t(double(zt) * t.inv_qt1);
-        const uint64_t Q = qf ? qf - 1 : 0;
+        const int64_t qt1 = int64_t(t.qt1);
+        int64_t Q = int64_t(double(zt) * t.inv_qt1);
+        const int64_t rem = int64_t(zt) - Q * qt1;
+        Q += (rem >= qt1) - (rem < 0);
         uint64_t mc = 0;
         int64_t br = 0;
         for (uint32_t i = 0; i < L; ++i) {
-            const uint64_t pq = Q * t.q[i] + mc;
+            const uint64_t pq = uint64_t(Q) * t.q[i] + mc;
             mc = pq >> 32;
-            const int64_t d = int64_t(uint32_t(z[i])) - int64_t(uint32_t(pq)) - br;
-            br = d < 0;
-            z[i] = int64_t(uint32_t(d));
+            const int64_t dd = int64_t(d[i]) - int64_t(uint32_t(pq)) - br;
+            br = dd < 0;
+            d[i] = uint32_t(dd);
         }
         top -= int64_t(mc) + br;
-        for (int it = 0; it < 3; ++it) {
-            bool ge = top > 0;
-            if (top == 0) {
-                ge = true;
-                for (int i = int(L) - 1; i >= 0; --i) {
-                    const uint32_t zi = uint32_t(z[i]), qi = t.q[i];
-                    if (zi != qi) {
-                        ge = zi > qi;
-                        break;
-                    }
+        bool ge = top > 0;
+        if (top == 0) {
+            ge = true;
+            for (int i = int(L) - 1; i >= 0; --i) {
+                if (d[i] != t.q[i]) {
+                    ge = d[i] > t.q[i];
+                    break;
                 }
             }
-            if (ge) {
-                int64_t b2 = 0;
-                for (uint32_t i = 0; i < L; ++i) {
-                    const int64_t d = int64_t(uint32_t(z[i])) - int64_t(t.q[i]) - b2;
-                    b2 = d < 0;
-                    z[i] = int64_t(uint32_t(d));
-                }
-                top -= b2;
+        }
+        if (ge) {
+            int64_t b2 = 0;
+            for (uint32_t i = 0; i < L; ++i) {
+                const int64_t dd = int64_t(d[i]) - int64_t(t.q[i]) - b2;
+                b2 = dd < 0;
+                d[i] = uint32_t(dd);
             }
         }
     }
     __syncthreads();
     uint32_t* dst = out + size_t(prod) * t.N * L;
-    for (uint32_t i = threadIdx.x; i < kCrtCoefs * L; i += kThreads) {
-        const uint32_t c = i / L, limb = i - c * L;
-        const uint32_t f = f0 + c;
-        const uint32_t n = (f >> t.n1) + t.N2 * (f & (t.N1 - 1));
-        dst[size_t(n) * L + limb] = uint32_t(lo[c * L1 + limb]);
+    const uint32_t w = threadIdx.x & 63;
+    if (w < L) {
+        for (uint32_t cc = threadIdx.x >> 6; cc < kCrtCoefs; cc += kCrtThreads / 64) {
+            const uint32_t f = f0 + cc;
+            const uint32_t n = (f >> t.n1) + t.N2 * (f & (t.N1 - 1));
+            dst[size_t(n) * L + w] = dig[cc * DS + w];
+        }
     }
 }
 
@@ -492,7 +606,8 @@ hipError_t dispatch_cols(const DevTables& t, const BatchArgs& args, hipStream_t 
 }
 
 size_t crt_smem_bytes(const DevTables& t) {
-    return size_t(kCrtCoefs) * (t.Kcrt + 16) + size_t(kCrtCoefs) * (t.L + 1) * 8;
+    // term rows (later reused for the digit rows) + int64 limb sums
+    return size_t(kCrtCoefs) * (t.Kcrt + 16) + size_t(t.L + 1) * kCrtCoefs * 8;
 }
 
 size_t enc_smem_bytes(const DevTables& t) { return size_t(kEncCoefs) * (t.Kenc + 16); }
@@ -521,7 +636,7 @@ hipError_t launch_forward_columns(const DevTables& t, const BatchArgs& args, hip
 
 hipError_t launch_middle(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     if (!shapes_ok(t, args)) return hipErrorInvalidValue;
-    const dim3 grid(t.P * (t.N1 / kMidBlocks), 1, args.batch);
+    const dim3 grid(t.N1 / kMidBlocks, args.batch, t.P);
     hipLaunchKernelGGL(k_middle, grid, dim3(kThreads), 0, stream, t, args.planes);
     return hipGetLastError();
 }
@@ -534,7 +649,7 @@ hipError_t launch_inverse_columns(const DevTables& t, const BatchArgs& args, hip
 hipError_t launch_crt(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     if (!shapes_ok(t, args)) return hipErrorInvalidValue;
     const dim3 grid(t.N / kCrtCoefs, 1, args.batch);
-    hipLaunchKernelGGL(k_crt, grid, dim3(kThreads), crt_smem_bytes(t), stream, t, args.planes, args.out);
+    hipLaunchKernelGGL(k_crt, grid, dim3(kCrtThreads), crt_smem_bytes(t), stream, t, args.planes, args.out);
     return hipGetLastError();
 }
 

@@ -10,15 +10,17 @@ namespace bcrt {
 // Device-resident constant tables of one plan, passed by value to every kernel.
 struct DevTables {
     uint32_t N, W, L, P, N1, n1, N2, Kenc, PTenc, Kcrt, Mcrt, qshift;
-    double inv_qt1;
+    uint64_t qt1;                  // (q >> qshift) + 1
+    double inv_qt1;                // 1 / qt1
     const uint32_t* primes;        // [P]
-    const uint32_t* mont_neg_inv;  // [P]
-    const uint2* zf;               // [P][N] (w, Shoup w')
-    const uint2* zi;               // [P][N]
-    const uint2* scale;            // [P]
+    const double* pd;              // [P][2]: p, 1/p
+    const double* zf;              // [P][N]: psi^brv(k) mod p
+    const double* zi;              // [P][N]: psi^-brv(k) mod p
+    const double* scale;           // [P]: N^-1 (P/p)^-1 mod p
+    const void* enc_prime;         // [P] x 24 bytes: off (f64), 1/p rounded down (f64), p, off mod 2^32
     const int8_t* wenc;            // [PTenc][32][Kenc]
     const uint64_t* enc_off;       // [P]
-    const uint64_t* enc_mu;        // [P]
+    const double* enc_inv;         // [P]: a double <= 1/p
     const int8_t* wcrt;            // [Mcrt][Kcrt]
     const int64_t* corr;           // [Mcrt/4]
     const double* invp;            // [P]

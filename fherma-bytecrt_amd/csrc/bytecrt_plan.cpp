@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <stdexcept>
 
 namespace bcrt {
@@ -144,10 +145,6 @@ bool is_prime32(uint32_t p) {
     return true;
 }
 
-uint32_t shoup_companion(uint32_t w, uint32_t p) {
-    return uint32_t((uint64_t(w) << 32) / p);
-}
-
 uint32_t bitrev(uint32_t x, unsigned nbits) {
     uint32_t r = 0;
     for (unsigned i = 0; i < nbits; ++i) r |= ((x >> i) & 1u) << (nbits - 1 - i);
@@ -237,14 +234,15 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
 
     pl.psi.resize(P);
     pl.crt_inverse.resize(P);
-    pl.mont_neg_inv.resize(P);
-    pl.scale.resize(2 * P);
+    pl.pd.resize(2 * P);
+    pl.scale.resize(P);
     pl.invp.resize(P);
     pl.enc_off.resize(P);
-    pl.enc_mu.resize(P);
+    pl.enc_inv.resize(P);
     pl.weights.assign(size_t(P + 1) * L, 0);
-    pl.zf.resize(size_t(P) * N * 2);
-    pl.zi.resize(size_t(P) * N * 2);
+    pl.zf.resize(size_t(P) * N);
+    pl.zi.resize(size_t(P) * N);
+    pl.enc_prime.assign(size_t(P) * 24, 0);
     pl.wenc.assign(size_t(g.PTenc) * 32 * g.Kenc, 0);
 
     std::vector<uint32_t> pw(N), pwi(N);
@@ -266,16 +264,11 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
         for (size_t i = 0; i < wj.size(); ++i) pl.weights[size_t(j) * L + i] = wj[i];
         pl.invp[j] = 1.0 / double(p);
 
-        // Montgomery: -p^-1 mod 2^32 by Newton iteration.
-        uint32_t inv = p;
-        for (int it = 0; it < 5; ++it) inv *= 2u - p * inv;
-        pl.mont_neg_inv[j] = 0u - inv;
-        // Pointwise products are Montgomery (x R^-1); the inverse transform leaves N.
-        const uint32_t r_mod = uint32_t((uint64_t{1} << 32) % p);
+        pl.pd[2 * j] = double(p);
+        pl.pd[2 * j + 1] = 1.0 / double(p);
+        // The inverse transform leaves N * c; t_j = c * crt_inverse_j.
         const uint32_t n_inv = pow_mod(N % p, p - 2, p);
-        const uint32_t sc = mulm(mulm(n_inv, r_mod, p), pl.crt_inverse[j], p);
-        pl.scale[2 * j] = sc;
-        pl.scale[2 * j + 1] = shoup_companion(sc, p);
+        pl.scale[j] = double(mulm(n_inv, pl.crt_inverse[j], p));
 
         // Twiddles: zf[k] = psi^brv(k), zi[k] = psi^-brv(k) (log_n-bit reversal).
         const uint32_t psi_inv = pow_mod(psi, p - 2, p);
@@ -286,14 +279,12 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
             a = mulm(a, psi, p);
             b = mulm(b, psi_inv, p);
         }
-        uint32_t* zf = pl.zf.data() + size_t(j) * N * 2;
-        uint32_t* zi = pl.zi.data() + size_t(j) * N * 2;
+        double* zf = pl.zf.data() + size_t(j) * N;
+        double* zi = pl.zi.data() + size_t(j) * N;
         for (uint32_t k = 0; k < N; ++k) {
             const uint32_t e = bitrev(k, g.log_n);
-            zf[2 * k] = pw[e];
-            zf[2 * k + 1] = shoup_companion(pw[e], p);
-            zi[2 * k] = pwi[e];
-            zi[2 * k + 1] = shoup_companion(pwi[e], p);
+            zf[k] = double(pw[e]);
+            zi[k] = double(pwi[e]);
         }
 
         // Encode weights: r_k == 2^(8k) mod p in (-p/2, p/2], four balanced digits.
@@ -318,7 +309,17 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
         if (c < 0) c += p;
         const uint64_t m = ((uint64_t{1} << 45) + p - 1) / p;
         pl.enc_off[j] = uint64_t(c) + m * p;
-        pl.enc_mu[j] = UINT64_MAX / p;
+        // (1/p)(1 - 2^-50) with both roundings stays below 1/p: u * enc_inv <= u/p.
+        pl.enc_inv[j] = (1.0 / double(p)) * (1.0 - std::ldexp(1.0, -50));
+        {
+            const double offd = double(pl.enc_off[j]);  // < 2^47: exact
+            const uint32_t off_lo = uint32_t(pl.enc_off[j]);
+            uint8_t* rec = pl.enc_prime.data() + size_t(j) * 24;
+            std::memcpy(rec, &offd, 8);
+            std::memcpy(rec + 8, &pl.enc_inv[j], 8);
+            std::memcpy(rec + 16, &p, 4);
+            std::memcpy(rec + 20, &off_lo, 4);
+        }
     }
     // Alpha weight (-P) mod q.
     {
@@ -362,6 +363,7 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
         uint64_t hi = a + 1 < L ? pl.q[a + 1] : 0;
         qt = ((lo | (hi << 32)) >> bsh) & 0xffffffffu;
     }
+    pl.qt1 = qt + 1;
     pl.inv_qt1 = 1.0 / double(qt + 1);
     return pl;
 }

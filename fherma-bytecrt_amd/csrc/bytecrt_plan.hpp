@@ -45,17 +45,19 @@ struct HostPlan {
     std::vector<uint32_t> psi;          // P: primitive 2N-th roots
     std::vector<uint32_t> crt_inverse;  // P: (P/p_j)^-1 mod p_j
     std::vector<uint32_t> weights;      // (P+1) x L: P_j mod q, last row (-P) mod q
-    std::vector<uint32_t> mont_neg_inv; // P: -p^-1 mod 2^32
-    std::vector<uint32_t> scale;        // P x 2: N^-1 * 2^32 * crt_inverse mod p, Shoup companion
-    std::vector<uint32_t> zf, zi;       // P x N x 2: (psi^{+-brv(k)}, Shoup companion)
+    std::vector<double> pd;             // P x 2: (p, 1/p) for the FP64 modular products
+    std::vector<double> scale;          // P: N^-1 * crt_inverse mod p
+    std::vector<double> zf, zi;         // P x N: psi^{+-brv(k)} mod p
     std::vector<int8_t> wenc;           // PTenc x 32 x Kenc
     std::vector<uint64_t> enc_off;      // P
-    std::vector<uint64_t> enc_mu;       // P: floor((2^64-1)/p)
+    std::vector<double> enc_inv;        // P: 1/p rounded down (FP64 quotient estimate)
+    std::vector<uint8_t> enc_prime;     // P x 24 bytes: {double off, double inv, u32 p, u32 off mod 2^32}
     std::vector<int8_t> wcrt;           // Mcrt x Kcrt
     std::vector<int64_t> corr;          // Mcrt/4 limb corrections
     std::vector<double> invp;           // P
     uint32_t qshift = 0;                // W - 32
-    double inv_qt1 = 0;                 // 1 / ((q >> qshift) + 1)
+    uint64_t qt1 = 0;                   // (q >> qshift) + 1
+    double inv_qt1 = 0;                 // 1 / qt1
 };
 
 // Throws std::invalid_argument (bad geometry or q) or std::runtime_error (no plan).
