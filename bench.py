@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--only-stage", choices=["encode", "fwd_cols", "middle", "inv_cols", "crt", "product"],
                     help="profiling mode: run one stage (or the whole product) --reps times on the batch")
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--sweep", action="store_true", help="time batch pipelining settings and exit")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="profiling mode: device-to-device copy of this many MiB, --reps times")
     return ap.parse_args()
@@ -142,6 +143,39 @@ def cpu_baseline(pkg, q, seconds: float):
             "sample": f"{done} full products N={N} W={W} (oracle C restatement, OpenMP) in {el:.1f} s on {cpu}"}
 
 
+def sweep_mode(args, torch):
+    """Batch time for each (chunk, streams) pipelining setting."""
+    sys.path.insert(0, str(ROOT))
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    q = pkg.datagen.modulus(W, 2026)
+    eng = pkg.Engine(N, W, q, max_batch=B)
+    a = device_operands(torch, B, N, q, 11)
+    b = device_operands(torch, B, N, q, 12)
+    c = torch.empty_like(a)
+    ref = torch.empty_like(a)
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()
+    eng.set_pipeline(B, 1)
+    eng.run_device(a, b, ref, B, stream=stream)
+    torch.cuda.synchronize()
+    rows = []
+    for chunk, streams in [(128, 1), (64, 2), (32, 2), (32, 4), (16, 2), (16, 4), (16, 8), (8, 4), (8, 8)]:
+        eng.set_pipeline(chunk, streams)
+        for graph in (False, True):
+            for _ in range(2):
+                eng.run_device(a, b, c, B, stream=stream, graph=graph)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                eng.run_device(a, b, c, B, stream=stream, graph=graph)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / 5
+            rows.append({"chunk": chunk, "streams": streams, "graph": graph, "ms": round(ms, 3),
+                         "exact": bool(torch.equal(c, ref))})
+            print(json.dumps(rows[-1]), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -152,6 +186,9 @@ def main():
 
     if args.only_stage or args.copy_probe_mib:
         profile_mode(args, torch)
+        return
+    if args.sweep:
+        sweep_mode(args, torch)
         return
     sys.path.insert(0, str(ROOT))
     import __graft_entry__ as ge

@@ -29,7 +29,7 @@ STAGE_ALL = 31
 EXPORTED = (
     "bcrt_create", "bcrt_destroy", "bcrt_run", "bcrt_run_device", "bcrt_run_device_graph",
     "bcrt_synchronize", "bcrt_last_error", "bcrt_geometry", "bcrt_plan_tables",
-    "bcrt_run_stages", "bcrt_planes_words", "bcrt_mfma_probe", "bcrt_plan_host",
+    "bcrt_run_stages", "bcrt_planes_words", "bcrt_mfma_probe", "bcrt_plan_host", "bcrt_set_pipeline",
 )
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -69,6 +69,8 @@ def load_library():
     lib.bcrt_planes_words.argtypes = [vp, ctypes.c_uint32]
     lib.bcrt_mfma_probe.restype = ctypes.c_int
     lib.bcrt_mfma_probe.argtypes = [vp, vp, vp]
+    lib.bcrt_set_pipeline.restype = ctypes.c_int
+    lib.bcrt_set_pipeline.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
     lib.bcrt_plan_host.restype = ctypes.c_int
     lib.bcrt_plan_host.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p,
                                    _u32p, _u32p, _u32p, _u32p, ctypes.c_char_p, ctypes.c_size_t]
@@ -174,6 +176,10 @@ class Engine:
         ptr = (lambda t: None if t is None else _dptr(t))
         self._check(lib.bcrt_run_stages(self._h, mask, ptr(a), ptr(b), ptr(c), ptr(planes), batch,
                                         _stream_handle(stream)))
+
+    def set_pipeline(self, chunk: int, streams: int):
+        """Split batches above `chunk` products over `streams` worker streams."""
+        self._check(load_library().bcrt_set_pipeline(self._h, chunk, streams))
 
     def synchronize(self):
         self._check(load_library().bcrt_synchronize(self._h))

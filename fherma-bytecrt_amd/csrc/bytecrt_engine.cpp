@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -33,6 +34,13 @@ struct bcrt_engine {
     hipGraph_t graph = nullptr;
     const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
     uint32_t graph_batch = 0;
+    // Batch pipelining: chunks of `pipe_chunk` products round-robin over worker
+    // streams so that stages of different chunks overlap on the device.
+    std::vector<hipStream_t> workers;
+    std::vector<hipEvent_t> worker_done;
+    hipEvent_t fork = nullptr;
+    uint32_t pipe_chunk = 16;
+    uint32_t pipe_streams = 4;
     std::string last_error;
 };
 
@@ -94,10 +102,56 @@ void drop_graph(bcrt_engine* e) {
 
 void release(bcrt_engine* e) {
     drop_graph(e);
+    for (auto ev : e->worker_done) (void)hipEventDestroy(ev);
+    e->worker_done.clear();
+    for (auto st : e->workers) (void)hipStreamDestroy(st);
+    e->workers.clear();
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    e->fork = nullptr;
     for (void* p : e->allocations) (void)hipFree(p);
     e->allocations.clear();
     if (e->stream) (void)hipStreamDestroy(e->stream);
     e->stream = nullptr;
+}
+
+void ensure_workers(bcrt_engine* e) {
+    while (e->workers.size() < e->pipe_streams) {
+        hipStream_t st = nullptr;
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "create worker stream");
+        e->workers.push_back(st);
+        hipEvent_t ev = nullptr;
+        check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "create worker event");
+        e->worker_done.push_back(ev);
+    }
+    if (!e->fork) check(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming), "create fork event");
+}
+
+// Enqueue the product for a batch on `origin`: one launch sequence when the
+// batch fits a chunk, else chunks spread over the worker streams, forked from
+// and joined back into `origin` (also valid inside a stream capture).
+void enqueue_batch(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
+                   uint32_t batch, hipStream_t origin) {
+    const auto& g = e->plan.g;
+    if (batch <= e->pipe_chunk || e->pipe_streams <= 1) {
+        const bcrt::BatchArgs args{d_a, d_b, d_c, e->planes, batch};
+        check(bcrt::launch_product(e->tables, args, origin), "launch product");
+        return;
+    }
+    ensure_workers(e);
+    const size_t io = size_t(g.N) * g.L, pl = size_t(2) * g.P * g.N;
+    const uint32_t used = std::min<uint32_t>(e->pipe_streams, (batch + e->pipe_chunk - 1) / e->pipe_chunk);
+    check(hipEventRecord(e->fork, origin), "fork");
+    for (uint32_t k = 0; k < used; ++k) check(hipStreamWaitEvent(e->workers[k], e->fork, 0), "fork wait");
+    uint32_t i = 0;
+    for (uint32_t c0 = 0; c0 < batch; c0 += e->pipe_chunk, ++i) {
+        const uint32_t n = std::min(e->pipe_chunk, batch - c0);
+        const bcrt::BatchArgs args{d_a + c0 * io, d_b + c0 * io, d_c + c0 * io, e->planes + c0 * pl, n};
+        check(bcrt::launch_product(e->tables, args, e->workers[i % used]), "launch chunk");
+    }
+    for (uint32_t k = 0; k < used; ++k) {
+        check(hipEventRecord(e->worker_done[k], e->workers[k]), "join record");
+        check(hipStreamWaitEvent(origin, e->worker_done[k], 0), "join wait");
+    }
 }
 
 }  // namespace
@@ -179,8 +233,7 @@ int bcrt_run_device(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, ui
     return guarded(e, [&] {
         require_batch(e, batch);
         if (!d_a || !d_b || !d_c) throw std::invalid_argument("ByteCRT null device pointer");
-        const bcrt::BatchArgs args{d_a, d_b, d_c, e->planes, batch};
-        check(bcrt::launch_product(e->tables, args, pick(e, stream)), "launch product");
+        enqueue_batch(e, d_a, d_b, d_c, batch, pick(e, stream));
     });
 }
 
@@ -196,12 +249,20 @@ int bcrt_run_device_graph(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d
         if (!hit) {
             drop_graph(e);
             // Capture on the engine's own stream, then launch on the caller's.
+            if (batch > e->pipe_chunk && e->pipe_streams > 1) ensure_workers(e);
             check(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-            const bcrt::BatchArgs args{d_a, d_b, d_c, e->planes, batch};
-            const hipError_t le = bcrt::launch_product(e->tables, args, e->stream);
+            std::string failure;
+            try {
+                enqueue_batch(e, d_a, d_b, d_c, batch, e->stream);
+            } catch (const std::exception& x) {
+                failure = x.what();
+            }
             hipGraph_t g = nullptr;
             const hipError_t ce = hipStreamEndCapture(e->stream, &g);
-            check(le, "launch product (capture)");
+            if (!failure.empty()) {
+                if (g) (void)hipGraphDestroy(g);
+                throw std::runtime_error(failure + " (during capture)");
+            }
             check(ce, "end capture");
             e->graph = g;
             check(hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0), "instantiate graph");
@@ -233,8 +294,7 @@ int bcrt_run(bcrt_engine* e, const uint32_t* a, const uint32_t* b, uint32_t* c, 
         const size_t bytes = size_t(batch) * words * sizeof(uint32_t);
         check(hipMemcpyAsync(da, a, bytes, hipMemcpyHostToDevice, e->stream), "copy a");
         check(hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, e->stream), "copy b");
-        const bcrt::BatchArgs args{da, db, dc, e->planes, batch};
-        check(bcrt::launch_product(e->tables, args, e->stream), "launch product");
+        enqueue_batch(e, da, db, dc, batch, e->stream);
         check(hipMemcpyAsync(c, dc, bytes, hipMemcpyDeviceToHost, e->stream), "copy c");
         check(hipStreamSynchronize(e->stream), "synchronize");
     });
@@ -317,4 +377,15 @@ extern "C" int bcrt_plan_host(uint32_t N, uint32_t W, uint32_t L, const uint32_t
         if (err && errlen) std::snprintf(err, errlen, "%s", x.what());
         return BCRT_ERUNTIME;
     }
+}
+
+extern "C" int bcrt_set_pipeline(bcrt_engine* e, uint32_t chunk, uint32_t streams) {
+    if (!e) return BCRT_EINVAL;
+    return guarded(e, [&] {
+        if (chunk == 0 || streams == 0 || streams > 16)
+            throw std::invalid_argument("ByteCRT pipeline needs chunk >= 1 and 1 <= streams <= 16");
+        drop_graph(e);
+        e->pipe_chunk = chunk;
+        e->pipe_streams = streams;
+    });
 }

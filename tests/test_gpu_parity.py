@@ -169,6 +169,16 @@ def test_batch_graph_and_whole_call(pkg, orc, gu):
         np.testing.assert_array_equal(gu.from_dev(out).reshape(B, N, e.L), batched)
     whole = e.run(a, b)
     np.testing.assert_array_equal(whole, batched)
+    # pipelined: chunks of one product over three worker streams, eager and graph
+    e.set_pipeline(1, 3)
+    for graph in (False, True, True):
+        out.zero_()
+        e.run_device(da, db, out, B, graph=graph)
+        np.testing.assert_array_equal(gu.from_dev(out).reshape(B, N, e.L), batched)
+    e.set_pipeline(3, 2)
+    out.zero_()
+    e.run_device(da, db, out, B, graph=True)
+    np.testing.assert_array_equal(gu.from_dev(out).reshape(B, N, e.L), batched)
 
 
 def test_fherma_mirror_roundtrip(pkg, orc, gu):
