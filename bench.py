@@ -34,6 +34,7 @@ def parse():
     ap.add_argument("--plain-calls", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-whole-call", action="store_true")
     ap.add_argument("--only-stage", choices=["encode", "fwd_cols", "middle", "inv_cols", "crt", "product"],
                     help="profiling mode: run one stage (or the whole product) --reps times on the batch")
     ap.add_argument("--reps", type=int, default=5)
@@ -113,6 +114,32 @@ def stage_bytes(P: int, L: int):
 def stage_ops(P: int, L: int, Kenc: int, Kcrt: int, Mcrt: int):
     """Algorithmic int8 MFMA ops (2 per MAC) per product for the GEMM kernels."""
     return {"encode": 2 * 2 * N * (4 * L) * (4 * P), "crt": 2 * N * (4 * (P + 1)) * (4 * L + 1)}
+
+
+def whole_call(eng, a, b, c_dev, torch):
+    """The reference's timing boundary: host buffers in, host buffers out
+    (pack, H2D, all GPU work, D2H, unpack), B=128 per call and B=1 per call."""
+    ah = a.cpu().numpy().view(np.uint32)
+    bh = b.cpu().numpy().view(np.uint32)
+    ref = c_dev.cpu().numpy().view(np.uint32)
+    eng.run(ah, bh)  # warm-up: allocates staging once
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ch = eng.run(ah, bh)
+        times.append(time.perf_counter() - t0)
+    exact = bool(np.array_equal(ch, ref))
+    one = []
+    for i in range(16):
+        t0 = time.perf_counter()
+        eng.run(ah[i], bh[i])
+        one.append(time.perf_counter() - t0)
+    ms = float(np.median(times) * 1e3)
+    ms1 = float(np.median(one) * 1e3)
+    return {"boundary": "whole call: host arrays in and out (PCIe and host copies included)",
+            "batch_ms_median": round(ms, 3), "batch_products_per_s": round(B / (ms / 1e3), 1),
+            "single_ms_median": round(ms1, 4), "single_products_per_s": round(1e3 / ms1, 1),
+            "batched_vs_plain": round((B / ms) / (1.0 / ms1), 3), "exact_vs_device": exact}
 
 
 def cpu_baseline(pkg, q, seconds: float):
@@ -273,6 +300,7 @@ def main():
     ntt_bytes = (sb["fwd_cols"] + sb["middle"] + sb["inv_cols"]) * B
     ntt_gbs = ntt_bytes / (ntt_ms * 1e-3) / 1e9
 
+    whole = None if args.no_whole_call else whole_call(eng, a, b, c, torch)
     cpu = None if args.no_cpu else cpu_baseline(pkg, q, args.cpu_seconds)
     line = {
         "metric": "exact products/sec, B=128 N=65,536 W=1,747 (batched vs plain); p50/p99 latency",
@@ -300,6 +328,7 @@ def main():
         "stages": stages,
         "ntt_hbm_frac": round(ntt_gbs / HBM_PEAK_GBS, 4),
         "roofline": roof,
+        "whole_call": whole,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

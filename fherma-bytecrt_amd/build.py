@@ -3,7 +3,8 @@
     python fherma-bytecrt_amd/build.py
 
 Outputs (git-ignored, shipped in-tree to the GPU host):
-    fherma-bytecrt_amd/lib/libbytecrt.so   C ABI (include/bytecrt.h) + fherma.h wrapper
+    fherma-bytecrt_amd/lib/libbytecrt.so      C ABI (include/bytecrt.h) + fherma.h wrapper
+    fherma-bytecrt_amd/lib/bytecrt_solution   point-directory runner (reference wire format)
 """
 from __future__ import annotations
 
@@ -20,6 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_engine.cpp", "fherma_solve.cpp"]
+# harness_main.cpp is linked separately into lib/bytecrt_solution.
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 # -ffp-contract=off: the exact FP64 modular arithmetic relies on every product
@@ -55,6 +57,10 @@ def build(verbose: bool = False) -> Path:
     os.replace(tmp, out)
     for o in objs:
         os.remove(o)
+    # The harness runner (reference wire format) links the library by rpath.
+    exe = LIB / "bytecrt_solution"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(CSRC / "harness_main.cpp"),
+          f"-L{LIB}", "-lbytecrt", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", str(exe)])
     if verbose:
         print(f"built {out}")
     return out
