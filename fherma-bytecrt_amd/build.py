@@ -20,7 +20,8 @@ LIB = PKG / "lib"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_engine.cpp", "fherma_solve.cpp"]
+SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_engine.cpp", "bytecrt_server.cpp",
+           "fherma_solve.cpp"]
 # harness_main.cpp is linked separately into lib/bytecrt_solution.
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
@@ -48,7 +49,7 @@ def build(verbose: bool = False) -> Path:
         if s.suffix == ".hip":
             cmd[1:1] = DEVICE
         else:
-            cmd[1:1] = [f"--offload-arch={ARCH}", "-x", "hip"] if src == "bytecrt_engine.cpp" else []
+            cmd[1:1] = [f"--offload-arch={ARCH}", "-x", "hip"] if src in ("bytecrt_engine.cpp", "bytecrt_server.cpp") else []
         _run(cmd)
         objs.append(str(o))
     out = LIB / "libbytecrt.so"

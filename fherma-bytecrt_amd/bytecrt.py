@@ -30,10 +30,20 @@ EXPORTED = (
     "bcrt_create", "bcrt_destroy", "bcrt_run", "bcrt_run_device", "bcrt_run_device_graph",
     "bcrt_synchronize", "bcrt_last_error", "bcrt_geometry", "bcrt_plan_tables",
     "bcrt_run_stages", "bcrt_planes_words", "bcrt_mfma_probe", "bcrt_plan_host", "bcrt_set_pipeline",
+    "bcrt_mix_create", "bcrt_mix_run", "bcrt_mix_last_error", "bcrt_mix_destroy",
+    "bcrt_server_create", "bcrt_server_submit", "bcrt_server_wait", "bcrt_server_stats",
+    "bcrt_server_last_error", "bcrt_server_destroy",
 )
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
 _LIB = None
+
+
+class _Group(ctypes.Structure):
+    """struct bcrt_group of bytecrt.h."""
+    _fields_ = [("engine", ctypes.c_void_p), ("d_a", ctypes.c_void_p), ("d_b", ctypes.c_void_p),
+                ("d_c", ctypes.c_void_p), ("batch", ctypes.c_uint32)]
 
 
 def load_library():
@@ -74,6 +84,27 @@ def load_library():
     lib.bcrt_plan_host.restype = ctypes.c_int
     lib.bcrt_plan_host.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p,
                                    _u32p, _u32p, _u32p, _u32p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.bcrt_mix_create.restype = ctypes.c_int
+    lib.bcrt_mix_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(_Group), ctypes.c_uint32,
+                                    ctypes.c_char_p, ctypes.c_size_t]
+    lib.bcrt_mix_run.restype = ctypes.c_int
+    lib.bcrt_mix_run.argtypes = [vp, vp, ctypes.c_int]
+    lib.bcrt_mix_last_error.restype = ctypes.c_char_p
+    lib.bcrt_mix_last_error.argtypes = [vp]
+    lib.bcrt_mix_destroy.argtypes = [vp]
+    lib.bcrt_server_create.restype = ctypes.c_int
+    lib.bcrt_server_create.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t, ctypes.c_uint32,
+                                       ctypes.c_char_p, ctypes.c_size_t]
+    lib.bcrt_server_submit.restype = ctypes.c_int
+    lib.bcrt_server_submit.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p,
+                                       _u32p, _u32p, _u64p]
+    lib.bcrt_server_wait.restype = ctypes.c_int
+    lib.bcrt_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+    lib.bcrt_server_stats.restype = ctypes.c_int
+    lib.bcrt_server_stats.argtypes = [vp, _u64p]
+    lib.bcrt_server_last_error.restype = ctypes.c_char_p
+    lib.bcrt_server_last_error.argtypes = [vp]
+    lib.bcrt_server_destroy.argtypes = [vp]
     _LIB = lib
     return lib
 
@@ -183,6 +214,103 @@ class Engine:
 
     def synchronize(self):
         self._check(load_library().bcrt_synchronize(self._h))
+
+
+class Mix:
+    """Products of several engines (different N, W, q) run as one launch per
+    stage.  ``groups`` is a list of (engine, d_a, d_b, d_c, batch)."""
+
+    def __init__(self, groups):
+        lib = load_library()
+        arr = (_Group * len(groups))()
+        for i, (eng, a, b, c, batch) in enumerate(groups):
+            arr[i] = _Group(eng._h.value, _dptr(a), _dptr(b), _dptr(c), batch)
+        self._engines = [g[0] for g in groups]  # keep the engines alive
+        handle = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = lib.bcrt_mix_create(ctypes.byref(handle), arr, len(groups), err, 512)
+        if rc != BCRT_OK:
+            _raise(rc, err.value.decode())
+        self._h = handle
+
+    def run(self, stream=None, graph: bool = False):
+        lib = load_library()
+        rc = lib.bcrt_mix_run(self._h, _stream_handle(stream), 1 if graph else 0)
+        if rc != BCRT_OK:
+            _raise(rc, lib.bcrt_mix_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().bcrt_mix_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+
+class Server:
+    """Continuous-batching product server over host buffers.
+
+    ``submit`` returns a ticket; ``wait`` blocks until that product is in the
+    output array and returns its submit-to-completion latency in seconds.
+    """
+
+    def __init__(self, hbm_budget_bytes: int = 8 << 30, max_wave: int = 128):
+        lib = load_library()
+        handle = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = lib.bcrt_server_create(ctypes.byref(handle), hbm_budget_bytes, max_wave, err, 512)
+        if rc != BCRT_OK:
+            _raise(rc, err.value.decode())
+        self._h = handle
+        self._live = {}  # ticket -> arrays kept alive until waited on
+
+    def _check(self, rc: int):
+        if rc != BCRT_OK:
+            _raise(rc, load_library().bcrt_server_last_error(self._h).decode())
+
+    def submit(self, N: int, W: int, q: int, a: np.ndarray, b: np.ndarray, c: np.ndarray | None = None):
+        L = (W + 31) // 32
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        if a.size != N * L or b.size != N * L:
+            raise RuntimeError("ByteCRT input coefficient count")
+        if c is None:
+            c = np.empty((N, L), dtype=np.uint32)
+        qw = limbs_of(q, L)
+        t = ctypes.c_uint64()
+        self._check(load_library().bcrt_server_submit(
+            self._h, N, W, L, qw.ctypes.data_as(_u32p), a.ctypes.data_as(_u32p), b.ctypes.data_as(_u32p),
+            c.ctypes.data_as(_u32p), ctypes.byref(t)))
+        self._live[t.value] = (a, b, c)
+        return t.value
+
+    def wait(self, ticket: int):
+        """(result array, latency seconds) of a submitted product."""
+        lat = ctypes.c_double()
+        rc = load_library().bcrt_server_wait(self._h, ticket, ctypes.byref(lat))
+        _, _, c = self._live.pop(ticket)
+        self._check(rc)
+        return c, lat.value
+
+    def stats(self) -> dict:
+        out = np.zeros(4, dtype=np.uint64)
+        self._check(load_library().bcrt_server_stats(self._h, out.ctypes.data_as(_u64p)))
+        return dict(completed=int(out[0]), waves=int(out[1]), max_wave=int(out[2]), plans=int(out[3]))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().bcrt_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
 
 
 def plan_host(N: int, W: int, q: int) -> dict:

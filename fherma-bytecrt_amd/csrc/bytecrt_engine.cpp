@@ -516,3 +516,120 @@ extern "C" int bcrt_set_pipeline(bcrt_engine* e, uint32_t chunk, uint32_t stream
         e->pipe_streams = streams;
     });
 }
+
+// ---------------------------------------------------------------------------
+// Mixed batches
+// ---------------------------------------------------------------------------
+struct bcrt_mix {
+    std::vector<bcrt::GroupDesc> groups;
+    bcrt::GroupDesc* d_groups = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::string last_error;
+};
+
+namespace {
+
+void release_mix(bcrt_mix* m) {
+    if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    if (m->d_groups) (void)hipFree(m->d_groups);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    m->graph_exec = nullptr;
+    m->graph = nullptr;
+    m->d_groups = nullptr;
+    m->stream = nullptr;
+}
+
+}  // namespace
+
+extern "C" int bcrt_mix_create(bcrt_mix** out, const bcrt_group* groups, uint32_t ngroups, char* err,
+                               size_t errlen) {
+    if (!out) return BCRT_EINVAL;
+    *out = nullptr;
+    auto* m = new bcrt_mix();
+    int rc = BCRT_OK;
+    try {
+        if (!groups || ngroups == 0 || ngroups > bcrt::kMaxGroups)
+            throw std::invalid_argument("ByteCRT mix needs 1.." + std::to_string(bcrt::kMaxGroups) + " groups");
+        uint32_t begin[5] = {0, 0, 0, 0, 0};
+        for (uint32_t k = 0; k < ngroups; ++k) {
+            const bcrt_group& g = groups[k];
+            if (!g.engine || !g.d_a || !g.d_b || !g.d_c)
+                throw std::invalid_argument("ByteCRT mix group " + std::to_string(k) + " has a null pointer");
+            for (uint32_t i = 0; i < k; ++i)
+                if (groups[i].engine == g.engine)
+                    throw std::invalid_argument("ByteCRT mix groups must use distinct engines");
+            require_batch(g.engine, g.batch);
+            bcrt::GroupDesc d{};
+            d.t = g.engine->tables;
+            d.args = bcrt::BatchArgs{g.d_a, g.d_b, g.d_c, g.engine->planes, g.batch};
+            for (int st = 0; st < 5; ++st) {
+                d.begin[st] = begin[st];
+                begin[st] += bcrt::group_workgroups(d.t, g.batch, st);
+            }
+            m->groups.push_back(d);
+        }
+        check(hipStreamCreate(&m->stream), "create mix stream");
+        check(hipMalloc(reinterpret_cast<void**>(&m->d_groups), sizeof(bcrt::GroupDesc) * ngroups),
+              "allocate group table");
+        check(hipMemcpy(m->d_groups, m->groups.data(), sizeof(bcrt::GroupDesc) * ngroups, hipMemcpyHostToDevice),
+              "upload group table");
+    } catch (const std::invalid_argument& x) {
+        rc = BCRT_EINVAL;
+        m->last_error = x.what();
+    } catch (const std::exception& x) {
+        rc = BCRT_ERUNTIME;
+        m->last_error = x.what();
+    }
+    if (rc != BCRT_OK) {
+        if (err && errlen) std::snprintf(err, errlen, "%s", m->last_error.c_str());
+        release_mix(m);
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return BCRT_OK;
+}
+
+extern "C" int bcrt_mix_run(bcrt_mix* m, void* stream, int graph) {
+    if (!m) return BCRT_EINVAL;
+    try {
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m->stream;
+        const uint32_t n = uint32_t(m->groups.size());
+        if (!graph) {
+            check(bcrt::launch_product_groups(m->d_groups, m->groups.data(), n, s), "launch mixed batch");
+        } else {
+            if (!m->graph_exec) {
+                check(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+                const hipError_t le = bcrt::launch_product_groups(m->d_groups, m->groups.data(), n, m->stream);
+                hipGraph_t g = nullptr;
+                const hipError_t ce = hipStreamEndCapture(m->stream, &g);
+                if (le != hipSuccess && g) (void)hipGraphDestroy(g);
+                check(le, "launch mixed batch (capture)");
+                check(ce, "end capture");
+                m->graph = g;
+                check(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0), "instantiate graph");
+            }
+            check(hipGraphLaunch(m->graph_exec, s), "graph launch");
+        }
+        m->last_error.clear();
+        return BCRT_OK;
+    } catch (const std::exception& x) {
+        m->last_error = x.what();
+        return BCRT_ERUNTIME;
+    }
+}
+
+extern "C" const char* bcrt_mix_last_error(const bcrt_mix* m) { return m ? m->last_error.c_str() : "null mix"; }
+
+extern "C" void bcrt_mix_destroy(bcrt_mix* m) {
+    if (!m) return;
+    release_mix(m);
+    delete m;
+}
+
+namespace bcrt {
+const DevTables& engine_tables(const bcrt_engine* e) { return e->tables; }
+}  // namespace bcrt

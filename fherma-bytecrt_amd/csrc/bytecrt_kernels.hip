@@ -19,6 +19,8 @@
 // kernels they are canonical u32 in [0, p).
 #include "bytecrt_kernels.hpp"
 
+#include <algorithm>
+
 namespace bcrt {
 namespace {
 
@@ -139,17 +141,21 @@ __device__ __forceinline__ uint32_t* plane_ptr(const DevTables& t, uint32_t* pla
 // bytes (x_byte ^ 0x80 = x_byte - 128); per prime
 //   x = sum_d 256^d C[4j+d] + 128 sum_k r_k  (mod p),  |sum_d 256^d C| < 2^45.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_encode(DevTables t, const uint32_t* __restrict__ a,
-                                                     const uint32_t* __restrict__ b,
-                                                     uint32_t* __restrict__ planes) {
-    extern __shared__ uint32_t enc_tile[];
+size_t enc_smem_bytes(const DevTables& t) {
+    return size_t(kEncCoefs) * (t.Kenc + 16) + sizeof(EncPrime) * kMaxPrimesCrt;
+}
+
+// One encode workgroup: tile bx of operand op of product prod.
+__device__ __forceinline__ void encode_wg(const DevTables& t, const uint32_t* __restrict__ a,
+                                          const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
+                                          uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem) {
+    uint32_t* enc_tile = reinterpret_cast<uint32_t*>(smem);
     // Per-prime reduction constants {off, 1/p rounded down, p, off mod 2^32}.
-    __shared__ EncPrime cst[kMaxPrimesCrt];
+    EncPrime* cst = reinterpret_cast<EncPrime*>(smem + size_t(kEncCoefs) * (t.Kenc + 16));
     for (uint32_t j = threadIdx.x; j < t.P; j += kThreads)
         cst[j] = reinterpret_cast<const EncPrime*>(t.enc_prime)[j];
-    const uint32_t op = blockIdx.y, prod = blockIdx.z;
     const uint32_t* in = (op ? b : a) + size_t(prod) * t.N * t.L;
-    const uint32_t f0 = blockIdx.x * kEncCoefs;
+    const uint32_t f0 = bx * kEncCoefs;
     const uint32_t row_words = t.Kenc / 4;
     const uint32_t stride_words = row_words + 4;
     {
@@ -239,16 +245,19 @@ constexpr uint32_t exchange_stride() {
 // 16 registers each; 4096/N1 columns per workgroup, all of one prime.  The
 // exchange between the two register rounds goes through LDS as FP64.
 // ---------------------------------------------------------------------------
+constexpr size_t kExchangeBytes = size_t(kThreads / 16) * exchange_stride<256>() * sizeof(double);
+
 template <int NLOG1>
-__global__ __launch_bounds__(kThreads) void k_fwd_cols(DevTables t, uint32_t* __restrict__ planes) {
+__device__ __forceinline__ void fwd_cols_wg(const DevTables& t, uint32_t* __restrict__ planes, uint32_t bx,
+                                            uint32_t op, uint32_t prod, double* s) {
     constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = kThreads / T, S = exchange_stride<N1>();
     constexpr int K2 = NLOG1 - 4;
-    __shared__ double s[K2 > 0 ? CPW * S : 1];
+    static_assert(K2 == 0 || CPW * S * sizeof(double) <= kExchangeBytes, "exchange buffer");
     const uint32_t wg_per_prime = t.N2 / CPW;
-    const uint32_t j = blockIdx.x / wg_per_prime;
-    const uint32_t r0 = (blockIdx.x % wg_per_prime) * CPW;
+    const uint32_t j = bx / wg_per_prime;
+    const uint32_t r0 = (bx % wg_per_prime) * CPW;
     const uint32_t cw = threadIdx.x / T, tau = threadIdx.x % T;
-    uint32_t* col = plane_ptr(t, planes, blockIdx.z, blockIdx.y, j) + size_t(r0 + cw) * N1;
+    uint32_t* col = plane_ptr(t, planes, prod, op, j) + size_t(r0 + cw) * N1;
     const double* z = t.zf + size_t(j) * t.N;
     const double p = t.pd[2 * j], pinv = t.pd[2 * j + 1];
     double v[16];
@@ -275,15 +284,15 @@ __global__ __launch_bounds__(kThreads) void k_fwd_cols(DevTables t, uint32_t* __
 }
 
 template <int NLOG1>
-__global__ __launch_bounds__(kThreads) void k_inv_cols(DevTables t, uint32_t* __restrict__ planes) {
+__device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __restrict__ planes, uint32_t bx,
+                                            uint32_t prod, double* s) {
     constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = kThreads / T, S = exchange_stride<N1>();
     constexpr int K2 = NLOG1 - 4;
-    __shared__ double s[K2 > 0 ? CPW * S : 1];
     const uint32_t wg_per_prime = t.N2 / CPW;
-    const uint32_t j = blockIdx.x / wg_per_prime;
-    const uint32_t r0 = (blockIdx.x % wg_per_prime) * CPW;
+    const uint32_t j = bx / wg_per_prime;
+    const uint32_t r0 = (bx % wg_per_prime) * CPW;
     const uint32_t cw = threadIdx.x / T, tau = threadIdx.x % T;
-    uint32_t* col = plane_ptr(t, planes, blockIdx.z, 0, j) + size_t(r0 + cw) * N1;
+    uint32_t* col = plane_ptr(t, planes, prod, 0, j) + size_t(r0 + cw) * N1;
     const double* z = t.zi + size_t(j) * t.N;
     const double p = t.pd[2 * j], pinv = t.pd[2 * j + 1];
     double v[16];
@@ -341,11 +350,10 @@ __device__ __forceinline__ void mid_store_lds(double* __restrict__ dst, const ui
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __restrict__ planes) {
-    __shared__ double sx[kMidBlocks * kMidS];
-    const uint32_t j = blockIdx.z;
-    const uint32_t b0 = blockIdx.x * kMidBlocks;
-    const uint32_t prod = blockIdx.y;
+__device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restrict__ planes, uint32_t bx,
+                                          uint32_t prod, uint32_t j, double* sx) {
+    static_assert(kMidBlocks * kMidS * sizeof(double) <= kExchangeBytes, "exchange buffer");
+    const uint32_t b0 = bx * kMidBlocks;
     const double p = t.pd[2 * j], pinv = t.pd[2 * j + 1];
     uint32_t* pa = plane_ptr(t, planes, prod, 0, j);
     uint32_t* pb = plane_ptr(t, planes, prod, 1, j);
@@ -425,9 +433,14 @@ __global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __re
 constexpr uint32_t kCrtThreads = 256;
 constexpr uint32_t kCrtMaxK = (4 * (kMaxPrimesCrt + 1) + 31) / 32;  // k steps of 32
 
-__global__ __launch_bounds__(kCrtThreads) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
-                                                     uint32_t* __restrict__ out) {
-    extern __shared__ __align__(16) uint8_t crt_smem[];
+size_t crt_smem_bytes(const DevTables& t) {
+    // term rows (later reused for the digit rows) + int64 limb sums
+    return size_t(kCrtCoefs) * (t.Kcrt + 16) + size_t(t.L + 1) * kCrtCoefs * 8;
+}
+
+__device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __restrict__ planes,
+                                       uint32_t* __restrict__ out, uint32_t bx, uint32_t prod,
+                                       uint8_t* crt_smem) {
     const uint32_t L = t.L, P = t.P;
     const uint32_t tstride = t.Kcrt + 16;  // bytes per term row (odd multiple of 16: conflict-free b128)
     const uint32_t tw = tstride / 4;
@@ -437,7 +450,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt(DevTables t, const uint32_t
     uint32_t* tmw = reinterpret_cast<uint32_t*>(crt_smem);
     uint32_t* dig = tmw;  // [64][DS], reuses the term rows once the GEMM is done
     int64_t* lo = reinterpret_cast<int64_t*>(crt_smem + kCrtCoefs * tstride);  // [L+1][64]
-    const uint32_t f0 = blockIdx.x * kCrtCoefs, prod = blockIdx.z;
+    const uint32_t f0 = bx * kCrtCoefs;
     const uint32_t* pa = planes + size_t(prod) * 2 * P * t.N + f0;
     {
         // lane c gathers four consecutive terms of its coefficient (coalesced per
@@ -583,14 +596,136 @@ __global__ void k_mfma_probe(const int8_t* a, const int8_t* b, int32_t* c) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Entry kernels.  Single-plan kernels take the plan by value and a 3-D grid;
+// group kernels take a table of (plan, batch) groups and a 1-D grid whose
+// workgroups are numbered group after group, so one launch per stage covers
+// products of different geometries and moduli.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_encode(DevTables t, const uint32_t* __restrict__ a,
+                                                     const uint32_t* __restrict__ b,
+                                                     uint32_t* __restrict__ planes) {
+    extern __shared__ __align__(16) uint8_t smem_enc[];
+    encode_wg(t, a, b, planes, blockIdx.x, blockIdx.y, blockIdx.z, smem_enc);
+}
+
+template <int NLOG1>
+__global__ __launch_bounds__(kThreads) void k_fwd_cols(DevTables t, uint32_t* __restrict__ planes) {
+    extern __shared__ __align__(16) uint8_t smem_fc[];
+    fwd_cols_wg<NLOG1>(t, planes, blockIdx.x, blockIdx.y, blockIdx.z, reinterpret_cast<double*>(smem_fc));
+}
+
+template <int NLOG1>
+__global__ __launch_bounds__(kThreads) void k_inv_cols(DevTables t, uint32_t* __restrict__ planes) {
+    extern __shared__ __align__(16) uint8_t smem_ic[];
+    inv_cols_wg<NLOG1>(t, planes, blockIdx.x, blockIdx.z, reinterpret_cast<double*>(smem_ic));
+}
+
+__global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __restrict__ planes) {
+    extern __shared__ __align__(16) uint8_t smem_mid[];
+    middle_wg(t, planes, blockIdx.x, blockIdx.y, blockIdx.z, reinterpret_cast<double*>(smem_mid));
+}
+
+__global__ __launch_bounds__(kCrtThreads) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
+                                                     uint32_t* __restrict__ out) {
+    extern __shared__ __align__(16) uint8_t smem_crt[];
+    crt_wg(t, planes, out, blockIdx.x, blockIdx.z, smem_crt);
+}
+
+// Workgroups of each stage for one group: (x, y, z) of the single-plan grid.
+__host__ __device__ inline uint32_t cols_per_wg(uint32_t n1) { return kThreads / ((1u << n1) / 16); }
+
+__host__ __device__ inline void stage_grid(const DevTables& t, uint32_t batch, int stage, uint32_t* x,
+                                           uint32_t* y, uint32_t* z) {
+    switch (stage) {
+        case 0: *x = t.N / kEncCoefs; *y = 2; *z = batch; break;
+        case 1: *x = t.P * (t.N2 / cols_per_wg(t.n1)); *y = 2; *z = batch; break;
+        case 2: *x = t.N1 / kMidBlocks; *y = batch; *z = t.P; break;
+        case 3: *x = t.P * (t.N2 / cols_per_wg(t.n1)); *y = 1; *z = batch; break;
+        default: *x = t.N / kCrtCoefs; *y = 1; *z = batch; break;
+    }
+}
+
+// Finds this workgroup's group and its (x, y, z) in that group's grid.
+__device__ __forceinline__ const GroupDesc* locate(const GroupDesc* __restrict__ g, uint32_t ng, int stage,
+                                                   uint32_t* x, uint32_t* y, uint32_t* z) {
+    uint32_t k = 0;
+    while (k + 1 < ng && blockIdx.x >= g[k + 1].begin[stage]) ++k;
+    const GroupDesc* d = g + k;
+    uint32_t gx, gy, gz;
+    stage_grid(d->t, d->args.batch, stage, &gx, &gy, &gz);
+    const uint32_t local = blockIdx.x - d->begin[stage];
+    *x = local % gx;
+    *y = (local / gx) % gy;
+    *z = local / (gx * gy);
+    return d;
+}
+
+__global__ __launch_bounds__(kThreads) void k_encode_groups(const GroupDesc* __restrict__ g, uint32_t ng) {
+    extern __shared__ __align__(16) uint8_t smem_enc_g[];
+    uint32_t x, y, z;
+    const GroupDesc* d = locate(g, ng, 0, &x, &y, &z);
+    encode_wg(d->t, d->args.a, d->args.b, d->args.planes, x, y, z, smem_enc_g);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fwd_cols_groups(const GroupDesc* __restrict__ g, uint32_t ng) {
+    extern __shared__ __align__(16) uint8_t smem_fc_g[];
+    uint32_t x, y, z;
+    const GroupDesc* d = locate(g, ng, 1, &x, &y, &z);
+    double* s = reinterpret_cast<double*>(smem_fc_g);
+    switch (d->t.n1) {
+        case 4: fwd_cols_wg<4>(d->t, d->args.planes, x, y, z, s); break;
+        case 5: fwd_cols_wg<5>(d->t, d->args.planes, x, y, z, s); break;
+        case 6: fwd_cols_wg<6>(d->t, d->args.planes, x, y, z, s); break;
+        case 7: fwd_cols_wg<7>(d->t, d->args.planes, x, y, z, s); break;
+        default: fwd_cols_wg<8>(d->t, d->args.planes, x, y, z, s); break;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_middle_groups(const GroupDesc* __restrict__ g, uint32_t ng) {
+    extern __shared__ __align__(16) uint8_t smem_mid_g[];
+    uint32_t x, y, z;
+    const GroupDesc* d = locate(g, ng, 2, &x, &y, &z);
+    middle_wg(d->t, d->args.planes, x, y, z, reinterpret_cast<double*>(smem_mid_g));
+}
+
+__global__ __launch_bounds__(kThreads) void k_inv_cols_groups(const GroupDesc* __restrict__ g, uint32_t ng) {
+    extern __shared__ __align__(16) uint8_t smem_ic_g[];
+    uint32_t x, y, z;
+    const GroupDesc* d = locate(g, ng, 3, &x, &y, &z);
+    double* s = reinterpret_cast<double*>(smem_ic_g);
+    switch (d->t.n1) {
+        case 4: inv_cols_wg<4>(d->t, d->args.planes, x, z, s); break;
+        case 5: inv_cols_wg<5>(d->t, d->args.planes, x, z, s); break;
+        case 6: inv_cols_wg<6>(d->t, d->args.planes, x, z, s); break;
+        case 7: inv_cols_wg<7>(d->t, d->args.planes, x, z, s); break;
+        default: inv_cols_wg<8>(d->t, d->args.planes, x, z, s); break;
+    }
+}
+
+__global__ __launch_bounds__(kCrtThreads) void k_crt_groups(const GroupDesc* __restrict__ g, uint32_t ng) {
+    extern __shared__ __align__(16) uint8_t smem_crt_g[];
+    uint32_t x, y, z;
+    const GroupDesc* d = locate(g, ng, 4, &x, &y, &z);
+    crt_wg(d->t, d->args.planes, d->args.out, x, z, smem_crt_g);
+}
+
+bool shapes_ok(const DevTables& t, const BatchArgs& args) {
+    return args.batch >= 1 && args.batch <= 65535 && t.N2 == 256 && t.N1 >= 16 && t.N1 <= 256 &&
+           t.N == t.N1 * t.N2 && t.N % kEncCoefs == 0 && t.Kenc % 32 == 0 && t.Kenc <= 256 &&
+           t.Kcrt % 32 == 0 && t.Mcrt % 32 == 0 && t.P >= 1 && t.P <= kMaxPrimesCrt && t.L >= 3 &&
+           t.L <= 64 && t.Kcrt >= 4 * (t.P + 1) && t.Mcrt >= 4 * t.L + 1 && t.Kenc >= 4 * t.L &&
+           t.qshift + 32 == t.W;
+}
+
 template <int NLOG1>
 hipError_t launch_cols(const DevTables& t, const BatchArgs& args, hipStream_t stream, bool forward) {
     constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = kThreads / T;
     const dim3 grid(t.P * (t.N2 / CPW), forward ? 2 : 1, args.batch);
     if (forward)
-        hipLaunchKernelGGL(k_fwd_cols<NLOG1>, grid, dim3(kThreads), 0, stream, t, args.planes);
+        hipLaunchKernelGGL(k_fwd_cols<NLOG1>, grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
     else
-        hipLaunchKernelGGL(k_inv_cols<NLOG1>, grid, dim3(kThreads), 0, stream, t, args.planes);
+        hipLaunchKernelGGL(k_inv_cols<NLOG1>, grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
     return hipGetLastError();
 }
 
@@ -603,20 +738,6 @@ hipError_t dispatch_cols(const DevTables& t, const BatchArgs& args, hipStream_t 
         case 8: return launch_cols<8>(t, args, stream, forward);
         default: return hipErrorInvalidValue;
     }
-}
-
-size_t crt_smem_bytes(const DevTables& t) {
-    // term rows (later reused for the digit rows) + int64 limb sums
-    return size_t(kCrtCoefs) * (t.Kcrt + 16) + size_t(t.L + 1) * kCrtCoefs * 8;
-}
-
-size_t enc_smem_bytes(const DevTables& t) { return size_t(kEncCoefs) * (t.Kenc + 16); }
-
-bool shapes_ok(const DevTables& t, const BatchArgs& args) {
-    return args.batch >= 1 && args.batch <= 65535 && t.N2 == 256 && t.N1 >= 16 && t.N1 <= 256 &&
-           t.N == t.N1 * t.N2 && t.N % kEncCoefs == 0 && t.Kenc % 32 == 0 && t.Kcrt % 32 == 0 &&
-           t.Mcrt % 32 == 0 && t.P >= 1 && t.P <= 128 && t.L >= 3 && t.L <= 64 &&
-           t.Kcrt >= 4 * (t.P + 1) && t.Mcrt >= 4 * t.L + 1 && t.Kenc >= 4 * t.L && t.qshift + 32 == t.W;
 }
 
 }  // namespace
@@ -637,7 +758,7 @@ hipError_t launch_forward_columns(const DevTables& t, const BatchArgs& args, hip
 hipError_t launch_middle(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     if (!shapes_ok(t, args)) return hipErrorInvalidValue;
     const dim3 grid(t.N1 / kMidBlocks, args.batch, t.P);
-    hipLaunchKernelGGL(k_middle, grid, dim3(kThreads), 0, stream, t, args.planes);
+    hipLaunchKernelGGL(k_middle, grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
     return hipGetLastError();
 }
 
@@ -667,10 +788,42 @@ hipError_t launch_mfma_probe(const int8_t* a, const int8_t* b, int32_t* c, hipSt
     return hipGetLastError();
 }
 
+uint32_t group_workgroups(const DevTables& t, uint32_t batch, int stage) {
+    uint32_t x, y, z;
+    stage_grid(t, batch, stage, &x, &y, &z);
+    return x * y * z;
+}
+
+hipError_t launch_product_groups(const GroupDesc* d_groups, const GroupDesc* h_groups, uint32_t ngroups,
+                                 hipStream_t stream) {
+    if (ngroups == 0 || ngroups > kMaxGroups) return hipErrorInvalidValue;
+    size_t enc = 0, crt = 0;
+    uint32_t total[5] = {0, 0, 0, 0, 0};
+    for (uint32_t k = 0; k < ngroups; ++k) {
+        const GroupDesc& g = h_groups[k];
+        if (!shapes_ok(g.t, g.args)) return hipErrorInvalidValue;
+        enc = std::max(enc, enc_smem_bytes(g.t));
+        crt = std::max(crt, crt_smem_bytes(g.t));
+        for (int st = 0; st < 5; ++st) {
+            if (g.begin[st] != total[st]) return hipErrorInvalidValue;
+            total[st] += group_workgroups(g.t, g.args.batch, st);
+        }
+    }
+    hipLaunchKernelGGL(k_encode_groups, dim3(total[0]), dim3(kThreads), enc, stream, d_groups, ngroups);
+    hipLaunchKernelGGL(k_fwd_cols_groups, dim3(total[1]), dim3(kThreads), kExchangeBytes, stream, d_groups, ngroups);
+    hipLaunchKernelGGL(k_middle_groups, dim3(total[2]), dim3(kThreads), kExchangeBytes, stream, d_groups, ngroups);
+    hipLaunchKernelGGL(k_inv_cols_groups, dim3(total[3]), dim3(kThreads), kExchangeBytes, stream, d_groups, ngroups);
+    hipLaunchKernelGGL(k_crt_groups, dim3(total[4]), dim3(kCrtThreads), crt, stream, d_groups, ngroups);
+    return hipGetLastError();
+}
+
 hipError_t configure_kernels() {
     // CRT may need more than the 64 KiB default dynamic LDS.
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(k_crt),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    for (const void* k : {reinterpret_cast<const void*>(k_crt), reinterpret_cast<const void*>(k_crt_groups)}) {
+        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace bcrt

@@ -39,6 +39,23 @@ struct BatchArgs {
     uint32_t batch;
 };
 
+// One group of a mixed launch: a plan and a batch of products under it.
+// begin[s] is the group's first workgroup in stage s (prefix sums over the
+// groups, in order; group_workgroups gives each group's count).
+constexpr uint32_t kMaxGroups = 64;
+struct GroupDesc {
+    DevTables t;
+    BatchArgs args;
+    uint32_t begin[5];
+};
+
+uint32_t group_workgroups(const DevTables& t, uint32_t batch, int stage);
+
+// Enqueue the five stages for all groups: one launch per stage covering every
+// product of every group.  d_groups is the device copy of h_groups.
+hipError_t launch_product_groups(const GroupDesc* d_groups, const GroupDesc* h_groups, uint32_t ngroups,
+                                 hipStream_t stream);
+
 // Enqueue the whole product pipeline for a batch on `stream` (no host sync).
 hipError_t launch_product(const DevTables& t, const BatchArgs& args, hipStream_t stream);
 

@@ -73,6 +73,50 @@ int bcrt_run_device_graph(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d
  * not depend on the setting. */
 int bcrt_set_pipeline(bcrt_engine* e, uint32_t chunk, uint32_t streams);
 
+/* ---- Mixed batches (products of different N, W, q in one launch per stage) ----
+ * No reference counterpart: the reference runs one fixed geometry per process
+ * (native-batched/solve.cu:163-164).  Each group names an engine (its plan
+ * and workspace; engines must be distinct) and a device batch for it.  The
+ * five stages each run as ONE launch whose workgroups cover every product of
+ * every group. */
+typedef struct bcrt_group {
+    bcrt_engine* engine;
+    const uint32_t* d_a;  /* [batch][N][L] device */
+    const uint32_t* d_b;
+    uint32_t* d_c;
+    uint32_t batch;       /* <= the engine's max_batch */
+} bcrt_group;
+
+typedef struct bcrt_mix bcrt_mix;
+
+int bcrt_mix_create(bcrt_mix** out, const bcrt_group* groups, uint32_t ngroups, char* err, size_t errlen);
+/* Enqueue the mixed batch on `stream` (hipStream_t, NULL = default); graph
+ * != 0 replays a hipGraph captured on first use. */
+int bcrt_mix_run(bcrt_mix* m, void* stream, int graph);
+const char* bcrt_mix_last_error(const bcrt_mix* m);
+void bcrt_mix_destroy(bcrt_mix* m);
+
+/* ---- Serving: continuous batching of single-product requests ----
+ * No reference counterpart (the reference serves one fixed-size batch per
+ * fherma_run call).  Requests of any geometry are admitted, in arrival
+ * order, into waves while the device arena (hbm_budget_bytes, holding each
+ * request's operands, result and residue planes) and pinned staging have
+ * room; each wave runs as one mixed launch per stage on one of two compute
+ * streams, so waves admitted while others are on the GPU overlap with them.
+ * Plan tables stay resident per (N, W, q) for the server's lifetime.
+ * a, b, c are host [N][L] buffers that must stay valid until the request is
+ * waited on. */
+typedef struct bcrt_server bcrt_server;
+int bcrt_server_create(bcrt_server** out, size_t hbm_budget_bytes, uint32_t max_wave, char* err, size_t errlen);
+int bcrt_server_submit(bcrt_server* s, uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, const uint32_t* a,
+                       const uint32_t* b, uint32_t* c, uint64_t* ticket);
+/* Blocks until the request is complete; latency_s = submit-to-completion. */
+int bcrt_server_wait(bcrt_server* s, uint64_t ticket, double* latency_s);
+/* out[0..3] = completed requests, waves launched, largest wave, resident plans. */
+int bcrt_server_stats(bcrt_server* s, uint64_t* out);
+const char* bcrt_server_last_error(const bcrt_server* s);
+void bcrt_server_destroy(bcrt_server* s);
+
 /* Waits for all work the engine enqueued on its own stream. */
 int bcrt_synchronize(bcrt_engine* e);
 

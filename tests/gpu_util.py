@@ -40,3 +40,7 @@ def from_planes(pos: np.ndarray, N1: int) -> np.ndarray:
     out = np.empty_like(pos)
     out[..., natural_of_position(N, N1)] = pos
     return out
+
+
+def sync():
+    torch.cuda.synchronize()
