@@ -136,10 +136,35 @@ def whole_call(eng, a, b, c_dev, torch):
         one.append(time.perf_counter() - t0)
     ms = float(np.median(times) * 1e3)
     ms1 = float(np.median(one) * 1e3)
-    return {"boundary": "whole call: host arrays in and out (PCIe and host copies included)",
-            "batch_ms_median": round(ms, 3), "batch_products_per_s": round(B / (ms / 1e3), 1),
-            "single_ms_median": round(ms1, 4), "single_products_per_s": round(1e3 / ms1, 1),
-            "batched_vs_plain": round((B / ms) / (1.0 / ms1), 3), "exact_vs_device": exact}
+    out = {"boundary": "whole call: host arrays in and out (PCIe and host copies included)",
+           "batch_ms_median": round(ms, 3), "batch_products_per_s": round(B / (ms / 1e3), 1),
+           "single_ms_median": round(ms1, 4), "single_products_per_s": round(1e3 / ms1, 1),
+           "batched_vs_plain": round((B / ms) / (1.0 / ms1), 3), "exact_vs_device": exact}
+    out["fherma_run"] = fherma_run_timing()
+    return out
+
+
+def fherma_run_timing():
+    """fherma_run through include/fherma.h at the reference harness's timing
+    boundary (std::vector operands in, owned std::vector out), measured by the
+    compiled fherma_call_bench in a child process: B=128 per call, and B=1
+    per call (the plain one-product-per-call form)."""
+    import subprocess
+    exe = ROOT / "fherma-bytecrt_amd" / "lib" / "fherma_call_bench"
+    res = {}
+    for batch, reps in ((B, 7), (1, 41)):
+        r = subprocess.run([str(exe), str(batch), str(N), str(W), "7", str(reps)], capture_output=True, text=True,
+                           timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError(f"fherma_call_bench B={batch} failed: {r.stderr.strip()[-400:]}")
+        res[batch] = json.loads(r.stdout.strip().splitlines()[-1])
+    bms, ms1 = res[B]["median_ms"], res[1]["median_ms"]
+    return {"boundary": "fherma_run call (native-batched/main.cpp:182-187): pack, H2D, GPU, D2H, owned vector",
+            "B128_ms_median": bms, "B128_ms_min": res[B]["min_ms"], "B128_products_per_s": round(B / bms * 1e3, 1),
+            "B1_ms_median": ms1, "B1_products_per_s": round(1e3 / ms1, 1),
+            "plain_serial_B128_ms": round(B * ms1, 2), "batched_vs_plain": round(B * ms1 / bms, 3),
+            "init_s": res[B]["init_s"], "exact": bool(res[B]["exact"] and res[1]["exact"]),
+            "reference_rtx_pro_6000": {"B128_ms": 89.648935, "B1_ms": 1.674401, "plain_serial_B128_ms": 291.519151}}
 
 
 def cpu_baseline(pkg, q, seconds: float):

@@ -20,8 +20,8 @@ LIB = PKG / "lib"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_engine.cpp", "bytecrt_server.cpp",
-           "fherma_solve.cpp"]
+SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_io.cpp", "bytecrt_engine.cpp",
+           "bytecrt_server.cpp", "fherma_solve.cpp"]
 # harness_main.cpp is linked separately into lib/bytecrt_solution.
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
@@ -59,9 +59,10 @@ def build(verbose: bool = False) -> Path:
     for o in objs:
         os.remove(o)
     # The harness runner (reference wire format) links the library by rpath.
-    exe = LIB / "bytecrt_solution"
-    _run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(CSRC / "harness_main.cpp"),
-          f"-L{LIB}", "-lbytecrt", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", str(exe)])
+    # So does the whole-call timer (fherma_run at the harness boundary).
+    for exe, src in (("bytecrt_solution", "harness_main.cpp"), ("fherma_call_bench", "fherma_call_bench.cpp")):
+        _run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(CSRC / src),
+              f"-L{LIB}", "-lbytecrt", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", str(LIB / exe)])
     if verbose:
         print(f"built {out}")
     return out

@@ -27,7 +27,7 @@ STAGE_ENCODE, STAGE_FWD_COLS, STAGE_MIDDLE, STAGE_INV_COLS, STAGE_CRT = 1, 2, 4,
 STAGE_ALL = 31
 
 EXPORTED = (
-    "bcrt_create", "bcrt_destroy", "bcrt_run", "bcrt_run_device", "bcrt_run_device_graph",
+    "bcrt_create", "bcrt_destroy", "bcrt_run", "bcrt_run_sink", "bcrt_run_device", "bcrt_run_device_graph",
     "bcrt_synchronize", "bcrt_last_error", "bcrt_geometry", "bcrt_plan_tables",
     "bcrt_run_stages", "bcrt_planes_words", "bcrt_mfma_probe", "bcrt_plan_host", "bcrt_set_pipeline",
     "bcrt_mix_create", "bcrt_mix_run", "bcrt_mix_last_error", "bcrt_mix_destroy",

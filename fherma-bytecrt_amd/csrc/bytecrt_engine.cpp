@@ -17,72 +17,11 @@
 #include <functional>
 #include <thread>
 
+#include "bytecrt_io.hpp"
 #include "bytecrt_kernels.hpp"
 #include "bytecrt_plan.hpp"
 
 namespace {
-
-// Fixed pool of host threads for large memcpys between pageable caller
-// buffers and pinned staging (one parallel_for at a time).
-class CopyPool {
-   public:
-    explicit CopyPool(unsigned n) {
-        for (unsigned i = 0; i < n; ++i) threads_.emplace_back([this, i] { loop(i); });
-    }
-    ~CopyPool() {
-        {
-            std::lock_guard<std::mutex> g(m_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto& t : threads_) t.join();
-    }
-    size_t size() const { return threads_.size() + 1; }
-    // Copies bytes in size() parts, the caller thread doing one of them.
-    void copy(void* dst, const void* src, size_t bytes) {
-        const size_t parts = size();
-        const size_t step = (bytes / parts + 4095) / 4096 * 4096;
-        auto part = [=](size_t k) {
-            const size_t b0 = std::min(bytes, k * step), b1 = std::min(bytes, (k + 1) * step);
-            if (b1 > b0) std::memcpy(static_cast<char*>(dst) + b0, static_cast<const char*>(src) + b0, b1 - b0);
-        };
-        {
-            std::lock_guard<std::mutex> g(m_);
-            job_ = part;
-            pending_ = threads_.size();
-            ++gen_;
-        }
-        cv_.notify_all();
-        part(0);
-        std::unique_lock<std::mutex> lk(m_);
-        done_cv_.wait(lk, [this] { return pending_ == 0; });
-    }
-
-   private:
-    void loop(unsigned i) {
-        uint64_t seen = 0;
-        for (;;) {
-            std::function<void(size_t)> job;
-            {
-                std::unique_lock<std::mutex> lk(m_);
-                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
-                seen = gen_;
-                job = job_;
-            }
-            job(i + 1);
-            std::lock_guard<std::mutex> g(m_);
-            if (--pending_ == 0) done_cv_.notify_one();
-        }
-    }
-    std::vector<std::thread> threads_;
-    std::mutex m_;
-    std::condition_variable cv_, done_cv_;
-    std::function<void(size_t)> job_;
-    size_t pending_ = 0;
-    uint64_t gen_ = 0;
-    bool stop_ = false;
-};
 
 }  // namespace
 
@@ -98,14 +37,9 @@ struct bcrt_engine {
     hipStream_t stream = nullptr;
     std::vector<void*> allocations;
     uint32_t* planes = nullptr;  // [max_batch][2][P][N]
-    // Whole-call staging (lazily allocated): two slots of `io_chunk`
-    // products, each with device a/b/c regions and pinned host a/b/c buffers.
-    uint32_t io_chunk = 0;
-    uint32_t* io = nullptr;              // device [2 slots][3][io_chunk][N][L]
-    uint32_t* pinned = nullptr;          // host   [2 slots][3][io_chunk][N][L]
-    hipStream_t h2d = nullptr, d2h = nullptr;
-    hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
-    std::unique_ptr<CopyPool> pool;
+    // Whole-call host I/O (pinned staging, copy pools, drain thread), created
+    // on the first whole call.
+    std::unique_ptr<bcrt::HostIo> hostio;
     // Captured graph (one key).
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -179,17 +113,7 @@ void drop_graph(bcrt_engine* e) {
 
 void release(bcrt_engine* e) {
     drop_graph(e);
-    if (e->h2d) (void)hipStreamSynchronize(e->h2d);
-    if (e->d2h) (void)hipStreamSynchronize(e->d2h);
-    for (int k = 0; k < 2; ++k)
-        for (hipEvent_t* ev : {&e->ev_in[k], &e->ev_done[k], &e->ev_out[k]})
-            if (*ev) (void)hipEventDestroy(*ev), *ev = nullptr;
-    if (e->h2d) (void)hipStreamDestroy(e->h2d);
-    if (e->d2h) (void)hipStreamDestroy(e->d2h);
-    e->h2d = e->d2h = nullptr;
-    if (e->pinned) (void)hipHostFree(e->pinned);
-    e->pinned = nullptr;
-    e->pool.reset();
+    e->hostio.reset();
     for (auto ev : e->worker_done) (void)hipEventDestroy(ev);
     e->worker_done.clear();
     for (auto st : e->workers) (void)hipStreamDestroy(st);
@@ -240,6 +164,25 @@ void enqueue_batch(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uin
         check(hipEventRecord(e->worker_done[k], e->workers[k]), "join record");
         check(hipStreamWaitEvent(origin, e->worker_done[k], 0), "join wait");
     }
+}
+
+}  // namespace
+
+namespace {
+
+int run_whole(bcrt_engine* e, const uint32_t* a, const uint32_t* b, uint32_t* c, uint32_t batch, bcrt_sink sink,
+              void* ctx) {
+    if (!e) return BCRT_EINVAL;
+    return guarded(e, [&] {
+        require_batch(e, batch);
+        if (!a || !b || (!c && !sink)) throw std::invalid_argument("ByteCRT null host pointer");
+        const auto& g = e->plan.g;
+        if (!e->hostio) e->hostio = std::make_unique<bcrt::HostIo>(size_t(g.N) * g.L, e->max_batch, e->stream);
+        e->hostio->run(a, b, batch, c, sink, ctx,
+                       [e](const uint32_t* da, const uint32_t* db, uint32_t* dc, uint32_t n, hipStream_t st) {
+                           enqueue_batch(e, da, db, dc, n, st);
+                       });
+    });
 }
 
 }  // namespace
@@ -364,67 +307,12 @@ int bcrt_run_device_graph(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d
 }
 
 int bcrt_run(bcrt_engine* e, const uint32_t* a, const uint32_t* b, uint32_t* c, uint32_t batch) {
-    if (!e) return BCRT_EINVAL;
-    return guarded(e, [&] {
-        require_batch(e, batch);
-        if (!a || !b || !c) throw std::invalid_argument("ByteCRT null host pointer");
-        const auto& g = e->plan.g;
-        const size_t words = size_t(g.N) * g.L;
-        if (!e->io) {
-            // Chunks of ~64 MiB of input per slot (at least one product).
-            const size_t per_product = 2 * words * sizeof(uint32_t);
-            e->io_chunk = uint32_t(std::max<size_t>(1, std::min<size_t>(e->max_batch, (size_t(64) << 20) / per_product)));
-            const size_t slot_words = 3 * size_t(e->io_chunk) * words;
-            void* p = nullptr;
-            check(hipMalloc(&p, 2 * slot_words * sizeof(uint32_t)), "allocate io buffers");
-            e->allocations.push_back(p);
-            e->io = static_cast<uint32_t*>(p);
-            check(hipHostMalloc(reinterpret_cast<void**>(&e->pinned), 2 * slot_words * sizeof(uint32_t),
-                                hipHostMallocDefault),
-                  "allocate pinned staging");
-            check(hipStreamCreateWithFlags(&e->h2d, hipStreamNonBlocking), "create h2d stream");
-            check(hipStreamCreateWithFlags(&e->d2h, hipStreamNonBlocking), "create d2h stream");
-            for (int k = 0; k < 2; ++k)
-                for (hipEvent_t* ev : {&e->ev_in[k], &e->ev_done[k], &e->ev_out[k]})
-                    check(hipEventCreateWithFlags(ev, hipEventDisableTiming), "create io event");
-            const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-            e->pool = std::make_unique<CopyPool>(std::min(15u, hw > 1 ? hw - 1 : 0u));
-        }
-        const size_t chunk = e->io_chunk, slot_words = 3 * chunk * words;
-        const uint32_t nchunks = uint32_t((batch + chunk - 1) / chunk);
-        std::vector<uint32_t> count(nchunks);
-        // Drains chunk k: wait for its device-to-host copy, then unpack.
-        auto drain = [&](uint32_t k) {
-            const int slot = int(k & 1);
-            check(hipEventSynchronize(e->ev_out[slot]), "wait output copy");
-            const uint32_t* src = e->pinned + slot * slot_words + 2 * chunk * words;
-            e->pool->copy(c + size_t(k) * chunk * words, src, size_t(count[k]) * words * sizeof(uint32_t));
-        };
-        for (uint32_t k = 0; k < nchunks; ++k) {
-            const int slot = int(k & 1);
-            const uint32_t n = uint32_t(std::min<size_t>(chunk, batch - size_t(k) * chunk));
-            count[k] = n;
-            const size_t bytes = size_t(n) * words * sizeof(uint32_t);
-            if (k >= 2) drain(k - 2);  // also frees this slot's staging and device regions
-            uint32_t* hin = e->pinned + slot * slot_words;
-            uint32_t* din = e->io + slot * slot_words;
-            e->pool->copy(hin, a + size_t(k) * chunk * words, bytes);
-            e->pool->copy(hin + chunk * words, b + size_t(k) * chunk * words, bytes);
-            check(hipMemcpyAsync(din, hin, bytes, hipMemcpyHostToDevice, e->h2d), "copy a");
-            check(hipMemcpyAsync(din + chunk * words, hin + chunk * words, bytes, hipMemcpyHostToDevice, e->h2d),
-                  "copy b");
-            check(hipEventRecord(e->ev_in[slot], e->h2d), "record input");
-            check(hipStreamWaitEvent(e->stream, e->ev_in[slot], 0), "compute waits input");
-            enqueue_batch(e, din, din + chunk * words, din + 2 * chunk * words, n, e->stream);
-            check(hipEventRecord(e->ev_done[slot], e->stream), "record compute");
-            check(hipStreamWaitEvent(e->d2h, e->ev_done[slot], 0), "copy waits compute");
-            check(hipMemcpyAsync(hin + 2 * chunk * words, din + 2 * chunk * words, bytes, hipMemcpyDeviceToHost,
-                                 e->d2h),
-                  "copy c");
-            check(hipEventRecord(e->ev_out[slot], e->d2h), "record output");
-        }
-        for (uint32_t k = nchunks > 2 ? nchunks - 2 : 0; k < nchunks; ++k) drain(k);
-    });
+    return run_whole(e, a, b, c, batch, nullptr, nullptr);
+}
+
+int bcrt_run_sink(bcrt_engine* e, const uint32_t* a, const uint32_t* b, uint32_t batch, bcrt_sink sink, void* ctx) {
+    if (!sink) return BCRT_EINVAL;
+    return run_whole(e, a, b, nullptr, batch, sink, ctx);
 }
 
 int bcrt_synchronize(bcrt_engine* e) {

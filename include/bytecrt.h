@@ -52,6 +52,15 @@ void bcrt_destroy(bcrt_engine* e);
  * complete.  batch <= max_batch. */
 int bcrt_run(bcrt_engine* e, const uint32_t* a, const uint32_t* b, uint32_t* c, uint32_t batch);
 
+/* Replaces the owned-output drain of the batched adapter
+ * (native-batched/solve.cu:138-155, which appends each finished chunk to the
+ * output std::vector): the same whole call, but the [batch][N][L] result is
+ * handed over in order as consecutive pieces, sink(ctx, words, nwords),
+ * called from one engine thread while the call runs.  The call returns after
+ * the last sink call. */
+typedef void (*bcrt_sink)(void* ctx, const uint32_t* words, size_t nwords);
+int bcrt_run_sink(bcrt_engine* e, const uint32_t* a, const uint32_t* b, uint32_t batch, bcrt_sink sink, void* ctx);
+
 /* Device-resident form of fherma_run's GPU work (core/solve.cu:401-430 without
  * the PCIe legs): a, b, c are device pointers ([batch][N][L] u32); the five
  * kernels are enqueued on `stream` (a hipStream_t; NULL selects the engine's

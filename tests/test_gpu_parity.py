@@ -181,6 +181,22 @@ def test_batch_graph_and_whole_call(pkg, orc, gu):
     np.testing.assert_array_equal(gu.from_dev(out).reshape(B, N, e.L), batched)
 
 
+def test_whole_call_multi_chunk(pkg, gu):
+    """The host whole call at N=65,536 with B=5: three chunks through the
+    staging ring, several pieces per copy, equal to the device-resident run."""
+    N, W, B = 65536, 1747, 5
+    q = pkg.datagen.modulus(W, 44)
+    e = pkg.Engine(N, W, q, max_batch=B)
+    a = np.stack([pkg.datagen.operands(N, q, 140 + i, edge_fraction=0.05) for i in range(B)])
+    b = np.stack([pkg.datagen.operands(N, q, 180 + i) for i in range(B)])
+    out = gu.empty_dev(B * N * e.L)
+    e.run_device(gu.to_dev(a), gu.to_dev(b), out, B)
+    want = gu.from_dev(out).reshape(B, N, e.L)
+    for _ in range(2):
+        np.testing.assert_array_equal(e.run(a, b), want)
+    np.testing.assert_array_equal(e.run(a[3], b[3]), want[3])
+
+
 def test_fherma_mirror_roundtrip(pkg, orc, gu):
     bc = pkg.bytecrt
     N, W, B = 8192, 220, 2

@@ -47,3 +47,21 @@ def test_runner_end_to_end(tmp_path, pkg, orc):
         got = read_answer(tmp_path, i, B, N, L)
         for k in range(B):
             np.testing.assert_array_equal(got[k], pl.product(a[k], b[k]))
+
+
+def test_call_bench_usage():
+    r = subprocess.run([str(PKG_DIR / "lib" / "fherma_call_bench")], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+@pytest.mark.gpu
+def test_fherma_run_multi_chunk():
+    """fherma_run at N=65,536 with B=5 moves three chunks through the staging
+    ring with several output pieces each; products 0 and 4 must equal their
+    B=1 runs, and the owned output must have exactly B*N*L words."""
+    import json
+    r = subprocess.run([str(PKG_DIR / "lib" / "fherma_call_bench"), "5", "65536", "1747", "3", "2"],
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    assert res["exact"] is True and len(res["ms"]) == 2
