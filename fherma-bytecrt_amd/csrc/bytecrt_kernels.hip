@@ -32,6 +32,7 @@ constexpr uint32_t kEncCoefs = 128;  // coefficients per encode workgroup
 constexpr uint32_t kCrtCoefs = 64;   // coefficients per CRT workgroup
 constexpr uint32_t kMidBlocks = 16;  // 256-element blocks per middle workgroup
 constexpr uint32_t kMaxPrimesCrt = 128;
+constexpr int kEncChains = 2;        // independent MFMA accumulator chains in encode
 
 // ---------------------------------------------------------------------------
 // Exact modular arithmetic in FP64, p in (2^30, 2^31).
@@ -146,6 +147,7 @@ size_t enc_smem_bytes(const DevTables& t) {
 }
 
 // One encode workgroup: tile bx of operand op of product prod.
+template <int KS>
 __device__ __forceinline__ void encode_wg(const DevTables& t, const uint32_t* __restrict__ a,
                                           const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
                                           uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem) {
@@ -155,28 +157,47 @@ __device__ __forceinline__ void encode_wg(const DevTables& t, const uint32_t* __
     for (uint32_t j = threadIdx.x; j < t.P; j += kThreads)
         cst[j] = reinterpret_cast<const EncPrime*>(t.enc_prime)[j];
     const uint32_t* in = (op ? b : a) + size_t(prod) * t.N * t.L;
-    const uint32_t f0 = bx * kEncCoefs;
+    // The tile is TR consecutive r (index mod 256) by TS consecutive s (index
+    // div 256), TS = min(32, N1): each s brings TR consecutive coefficients
+    // (TR * L contiguous input words), each (prime, r) gets TS consecutive
+    // plane positions.  Tile row m is (r0 + m / TS, s0 + m % TS).
+    const uint32_t ts_log = min(5u, t.n1), TS = 1u << ts_log, TR = kEncCoefs >> ts_log;
+    const uint32_t rtiles = t.N2 / TR;
+    const uint32_t r0 = (bx % rtiles) * TR, s0 = (bx / rtiles) * TS;
     const uint32_t row_words = t.Kenc / 4;
     const uint32_t stride_words = row_words + 4;
     {
-        // Each wave loads one coefficient row per step (lanes over limbs); all
-        // 32 loads of a thread are issued before the first LDS store.
-        constexpr uint32_t kRowsPerStep = kThreads / 64;
-        constexpr uint32_t kSteps = kEncCoefs / kRowsPerStep;
-        const uint32_t w = threadIdx.x & 63, c0 = threadIdx.x >> 6;
+        // Word v of the tile: group gi = v / G (one s), offset o = v % G within
+        // the group's TR * L contiguous words.  Division by the magic
+        // ceil(2^32 / d) is exact for v * (magic * d - 2^32) < 2^32 (v < 2^13).
+        const uint32_t G = TR * t.L, total = kEncCoefs * t.L;
+        const uint32_t mg = 0xFFFFFFFFu / G + 1, ml = 0xFFFFFFFFu / t.L + 1;
+        const uint32_t* src = in + (size_t(r0) + size_t(t.N2) * s0) * t.L;
+        const uint32_t group_stride = t.N2 * t.L;  // 32-bit offsets: N * L < 2^22
+        constexpr uint32_t kSteps = kEncCoefs * 64 / kThreads;  // L <= 64
+        // Loads are unconditional (past the end a thread re-reads the tile's
+        // last word) so that all of them are in flight before the first use.
         uint32_t regs[kSteps];
 #pragma unroll
         for (uint32_t it = 0; it < kSteps; ++it) {
-            const uint32_t f = f0 + c0 + kRowsPerStep * it;
-            const uint32_t n = (f >> t.n1) + t.N2 * (f & (t.N1 - 1));
-            regs[it] = w < t.L ? in[size_t(n) * t.L + w] : 0u;
+            const uint32_t v = min(threadIdx.x + kThreads * it, total - 1);
+            const uint32_t gi = __umulhi(v, mg);
+            regs[it] = src[gi * group_stride + (v - gi * G)];
         }
         // Bytes are stored biased (x ^ 0x80 = x - 128) for the signed MFMA.
-        if (w < row_words) {
 #pragma unroll
-            for (uint32_t it = 0; it < kSteps; ++it)
-                enc_tile[(c0 + kRowsPerStep * it) * stride_words + w] = regs[it] ^ 0x80808080u;
+        for (uint32_t it = 0; it < kSteps; ++it) {
+            const uint32_t v = threadIdx.x + kThreads * it;
+            if (v < total) {
+                const uint32_t gi = __umulhi(v, mg), o = v - gi * G;
+                const uint32_t ro = __umulhi(o, ml), limb = o - ro * t.L;
+                enc_tile[((ro << ts_log) + gi) * stride_words + limb] = regs[it] ^ 0x80808080u;
+            }
         }
+        // Padding limbs L .. Kenc/4 - 1 hold biased zero bytes.
+        const uint32_t pad = row_words - t.L;
+        for (uint32_t i = threadIdx.x; i < kEncCoefs * pad; i += kThreads)
+            enc_tile[(i / pad) * stride_words + t.L + i % pad] = 0x80808080u;
     }
     __syncthreads();
 
@@ -184,44 +205,69 @@ __device__ __forceinline__ void encode_wg(const DevTables& t, const uint32_t* __
     const uint32_t row = lane & 31, kh = lane >> 5;
     const uint8_t* tile_bytes = reinterpret_cast<const uint8_t*>(enc_tile);
     const uint32_t stride_bytes = stride_words * 4;
-    constexpr uint32_t kMaxKSteps = 8;  // Kenc <= 256
-    const uint32_t ksteps = t.Kenc / 32;
-    for (uint32_t pt = wave; pt < t.PTenc; pt += 4) {
-        // The tile's weight fragments for every k step, loaded once.
+    uint32_t* out = plane_ptr(t, planes, prod, op, 0);  // prime j at out + j * N (P * N < 2^32)
+    // Wave w takes the coefficient sub-tiles 2 (w & 1) and 2 (w & 1) + 1 (their
+    // data fragments stay in registers) and weight tiles pt = w >> 1, +2, ...;
+    // the two sub-tiles accumulate in independent chains.
+    const uint32_t c0 = 2 * (wave & 1);
+    for (uint32_t pt = wave >> 1; pt < t.PTenc; pt += 2) {
         const int8_t* wrow = t.wenc + (size_t(pt) * 32 + row) * t.Kenc + 16 * kh;
-        v4i A[kMaxKSteps];
+        v4i A[KS];
 #pragma unroll
-        for (uint32_t s2 = 0; s2 < kMaxKSteps; ++s2)
-            A[s2] = s2 < ksteps ? *reinterpret_cast<const v4i*>(wrow + 32 * s2) : v4i{};
+        for (int s2 = 0; s2 < KS; ++s2) A[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
+        {
+            v16i acc[kEncChains];
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) {
-            v16i acc = v16i{};
-            const uint8_t* brow = tile_bytes + (ct * 32 + row) * stride_bytes + 16 * kh;
+            for (int c = 0; c < kEncChains; ++c) acc[c] = v16i{};
 #pragma unroll
-            for (uint32_t s2 = 0; s2 < kMaxKSteps; ++s2) {
-                if (s2 < ksteps) {
-                    const v4i B = *reinterpret_cast<const v4i*>(brow + 32 * s2);
-                    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[s2], B, acc, 0, 0, 0);
-                }
+            for (int s2 = 0; s2 < KS; ++s2) {
+                v4i B[kEncChains];
+#pragma unroll
+                for (int c = 0; c < kEncChains; ++c)
+                    B[c] = *reinterpret_cast<const v4i*>(tile_bytes + ((c0 + c) * 32 + row) * stride_bytes +
+                                                         16 * kh + 32 * s2);
+#pragma unroll
+                for (int c = 0; c < kEncChains; ++c)
+                    acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[s2], B[c], acc[c], 0, 0, 0);
             }
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const uint32_t j = pt * 8 + 2 * g + kh;
-                if (j < t.P) {
-                    const EncPrime c = cst[j];
-                    // digit sums |C| < 2^22: the two byte pairs fit int32
-                    const int32_t lo = acc[4 * g] + acc[4 * g + 1] * 256;
-                    const int32_t hi = acc[4 * g + 2] + acc[4 * g + 3] * 256;
-                    // u = lo + 2^16 hi + off, 0 <= u < 2^47: exact in FP64; c.inv <= 1/p
-                    // makes qd = floor(u/p) or one less, so r = u - qd p is in [0, 2p).
-                    const double ud = fma(double(hi), 65536.0, double(lo)) + c.off;
-                    const uint32_t qd = uint32_t(ud * c.inv);
-                    const uint32_t u32 = uint32_t(lo) + (uint32_t(hi) << 16) + c.off_lo;
-                    const uint32_t r = u32 - qd * c.p;
-                    plane_ptr(t, planes, prod, op, j)[f0 + ct * 32 + row] = min(r, r - c.p);
+            for (int c = 0; c < kEncChains; ++c) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const uint32_t j = pt * 8 + 2 * g + kh;
+                    if (j < t.P) {
+                        const EncPrime cp = cst[j];
+                        // digit sums |C| < 2^22: the two byte pairs fit int32
+                        const int32_t lo = acc[c][4 * g] + acc[c][4 * g + 1] * 256;
+                        const int32_t hi = acc[c][4 * g + 2] + acc[c][4 * g + 3] * 256;
+                        // u = lo + 2^16 hi + off, 0 <= u < 2^47: exact in FP64; cp.inv <= 1/p
+                        // makes qd = floor(u/p) or one less, so r = u - qd p is in [0, 2p).
+                        const double ud = fma(double(hi), 65536.0, double(lo)) + cp.off;
+                        const uint32_t qd = uint32_t(ud * cp.inv);
+                        const uint32_t u32 = uint32_t(lo) + (uint32_t(hi) << 16) + cp.off_lo;
+                        const uint32_t r = u32 - qd * cp.p;
+                        const uint32_t m = (c0 + c) * 32 + row;
+                        out[j * t.N + (r0 + (m >> ts_log)) * t.N1 + s0 + (m & (TS - 1))] = min(r, r - cp.p);
+                    }
                 }
             }
         }
+    }
+}
+
+// Encode with the k-step count (Kenc / 32, 1..8) as a compile-time constant.
+__device__ __forceinline__ void encode_dispatch(const DevTables& t, const uint32_t* __restrict__ a,
+                                                const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
+                                                uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem) {
+    switch (t.Kenc / 32) {
+        case 1: encode_wg<1>(t, a, b, planes, bx, op, prod, smem); break;
+        case 2: encode_wg<2>(t, a, b, planes, bx, op, prod, smem); break;
+        case 3: encode_wg<3>(t, a, b, planes, bx, op, prod, smem); break;
+        case 4: encode_wg<4>(t, a, b, planes, bx, op, prod, smem); break;
+        case 5: encode_wg<5>(t, a, b, planes, bx, op, prod, smem); break;
+        case 6: encode_wg<6>(t, a, b, planes, bx, op, prod, smem); break;
+        case 7: encode_wg<7>(t, a, b, planes, bx, op, prod, smem); break;
+        default: encode_wg<8>(t, a, b, planes, bx, op, prod, smem); break;
     }
 }
 
@@ -596,42 +642,6 @@ __global__ void k_mfma_probe(const int8_t* a, const int8_t* b, int32_t* c) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// Entry kernels.  Single-plan kernels take the plan by value and a 3-D grid;
-// group kernels take a table of (plan, batch) groups and a 1-D grid whose
-// workgroups are numbered group after group, so one launch per stage covers
-// products of different geometries and moduli.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_encode(DevTables t, const uint32_t* __restrict__ a,
-                                                     const uint32_t* __restrict__ b,
-                                                     uint32_t* __restrict__ planes) {
-    extern __shared__ __align__(16) uint8_t smem_enc[];
-    encode_wg(t, a, b, planes, blockIdx.x, blockIdx.y, blockIdx.z, smem_enc);
-}
-
-template <int NLOG1>
-__global__ __launch_bounds__(kThreads) void k_fwd_cols(DevTables t, uint32_t* __restrict__ planes) {
-    extern __shared__ __align__(16) uint8_t smem_fc[];
-    fwd_cols_wg<NLOG1>(t, planes, blockIdx.x, blockIdx.y, blockIdx.z, reinterpret_cast<double*>(smem_fc));
-}
-
-template <int NLOG1>
-__global__ __launch_bounds__(kThreads) void k_inv_cols(DevTables t, uint32_t* __restrict__ planes) {
-    extern __shared__ __align__(16) uint8_t smem_ic[];
-    inv_cols_wg<NLOG1>(t, planes, blockIdx.x, blockIdx.z, reinterpret_cast<double*>(smem_ic));
-}
-
-__global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __restrict__ planes) {
-    extern __shared__ __align__(16) uint8_t smem_mid[];
-    middle_wg(t, planes, blockIdx.x, blockIdx.y, blockIdx.z, reinterpret_cast<double*>(smem_mid));
-}
-
-__global__ __launch_bounds__(kCrtThreads) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
-                                                     uint32_t* __restrict__ out) {
-    extern __shared__ __align__(16) uint8_t smem_crt[];
-    crt_wg(t, planes, out, blockIdx.x, blockIdx.z, smem_crt);
-}
-
 // Workgroups of each stage for one group: (x, y, z) of the single-plan grid.
 __host__ __device__ inline uint32_t cols_per_wg(uint32_t n1) { return kThreads / ((1u << n1) / 16); }
 
@@ -646,18 +656,87 @@ __host__ __device__ inline void stage_grid(const DevTables& t, uint32_t batch, i
     }
 }
 
+// The dispatcher deals workgroups to the 8 XCDs round-robin (id % 8), and each
+// XCD has its own L2.  Renumbering so that XCD k runs the k-th contiguous
+// eighth of the tiles keeps neighbouring tiles -- which share cache lines of
+// their operands and the prime's twiddles -- in one L2 at about the same time.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t id, uint32_t total) {
+    const uint32_t per = total / 8, full = per * 8;
+    return id < full ? (id % 8) * per + id / 8 : id;
+}
+
+// (x, y, z) of tile `tile` in a stage grid of width gx and height gy.
+__device__ __forceinline__ void tile_xyz(uint32_t tile, uint32_t gx, uint32_t gy, uint32_t* x, uint32_t* y,
+                                         uint32_t* z) {
+    *x = tile % gx;
+    *y = (tile / gx) % gy;
+    *z = tile / (gx * gy);
+}
+
+// This workgroup's (x, y, z) in the single-plan grid of `stage` (1-D launch).
+__device__ __forceinline__ void single_xyz(const DevTables& t, int stage, uint32_t* x, uint32_t* y, uint32_t* z) {
+    uint32_t gx, gy, gz;
+    stage_grid(t, 1, stage, &gx, &gy, &gz);
+    stage_grid(t, gridDim.x / (gx * gy * gz), stage, &gx, &gy, &gz);  // the launch covers whole products
+    tile_xyz(xcd_tile(blockIdx.x, gridDim.x), gx, gy, x, y, z);
+}
+
+// ---------------------------------------------------------------------------
+// Entry kernels, all on 1-D grids of XCD-renumbered tiles.  Single-plan
+// kernels take the plan by value; group kernels take a table of (plan, batch)
+// groups whose tiles are numbered group after group, so one launch per stage
+// covers products of different geometries and moduli.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_encode(DevTables t, const uint32_t* __restrict__ a,
+                                                     const uint32_t* __restrict__ b,
+                                                     uint32_t* __restrict__ planes) {
+    extern __shared__ __align__(16) uint8_t smem_enc[];
+    uint32_t x, y, z;
+    single_xyz(t, 0, &x, &y, &z);
+    encode_dispatch(t, a, b, planes, x, y, z, smem_enc);
+}
+
+template <int NLOG1>
+__global__ __launch_bounds__(kThreads) void k_fwd_cols(DevTables t, uint32_t* __restrict__ planes) {
+    extern __shared__ __align__(16) uint8_t smem_fc[];
+    uint32_t x, y, z;
+    single_xyz(t, 1, &x, &y, &z);
+    fwd_cols_wg<NLOG1>(t, planes, x, y, z, reinterpret_cast<double*>(smem_fc));
+}
+
+template <int NLOG1>
+__global__ __launch_bounds__(kThreads) void k_inv_cols(DevTables t, uint32_t* __restrict__ planes) {
+    extern __shared__ __align__(16) uint8_t smem_ic[];
+    uint32_t x, y, z;
+    single_xyz(t, 3, &x, &y, &z);
+    inv_cols_wg<NLOG1>(t, planes, x, z, reinterpret_cast<double*>(smem_ic));
+}
+
+__global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __restrict__ planes) {
+    extern __shared__ __align__(16) uint8_t smem_mid[];
+    uint32_t x, y, z;
+    single_xyz(t, 2, &x, &y, &z);
+    middle_wg(t, planes, x, y, z, reinterpret_cast<double*>(smem_mid));
+}
+
+__global__ __launch_bounds__(kCrtThreads) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
+                                                     uint32_t* __restrict__ out) {
+    extern __shared__ __align__(16) uint8_t smem_crt[];
+    uint32_t x, y, z;
+    single_xyz(t, 4, &x, &y, &z);
+    crt_wg(t, planes, out, x, z, smem_crt);
+}
+
 // Finds this workgroup's group and its (x, y, z) in that group's grid.
 __device__ __forceinline__ const GroupDesc* locate(const GroupDesc* __restrict__ g, uint32_t ng, int stage,
                                                    uint32_t* x, uint32_t* y, uint32_t* z) {
     uint32_t k = 0;
-    while (k + 1 < ng && blockIdx.x >= g[k + 1].begin[stage]) ++k;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    while (k + 1 < ng && tile >= g[k + 1].begin[stage]) ++k;
     const GroupDesc* d = g + k;
     uint32_t gx, gy, gz;
     stage_grid(d->t, d->args.batch, stage, &gx, &gy, &gz);
-    const uint32_t local = blockIdx.x - d->begin[stage];
-    *x = local % gx;
-    *y = (local / gx) % gy;
-    *z = local / (gx * gy);
+    tile_xyz(tile - d->begin[stage], gx, gy, x, y, z);
     return d;
 }
 
@@ -665,7 +744,7 @@ __global__ __launch_bounds__(kThreads) void k_encode_groups(const GroupDesc* __r
     extern __shared__ __align__(16) uint8_t smem_enc_g[];
     uint32_t x, y, z;
     const GroupDesc* d = locate(g, ng, 0, &x, &y, &z);
-    encode_wg(d->t, d->args.a, d->args.b, d->args.planes, x, y, z, smem_enc_g);
+    encode_dispatch(d->t, d->args.a, d->args.b, d->args.planes, x, y, z, smem_enc_g);
 }
 
 __global__ __launch_bounds__(kThreads) void k_fwd_cols_groups(const GroupDesc* __restrict__ g, uint32_t ng) {
@@ -721,7 +800,7 @@ bool shapes_ok(const DevTables& t, const BatchArgs& args) {
 template <int NLOG1>
 hipError_t launch_cols(const DevTables& t, const BatchArgs& args, hipStream_t stream, bool forward) {
     constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = kThreads / T;
-    const dim3 grid(t.P * (t.N2 / CPW), forward ? 2 : 1, args.batch);
+    const dim3 grid(t.P * (t.N2 / CPW) * (forward ? 2 : 1) * args.batch);
     if (forward)
         hipLaunchKernelGGL(k_fwd_cols<NLOG1>, grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
     else
@@ -744,7 +823,7 @@ hipError_t dispatch_cols(const DevTables& t, const BatchArgs& args, hipStream_t 
 
 hipError_t launch_encode(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     if (!shapes_ok(t, args)) return hipErrorInvalidValue;
-    const dim3 grid(t.N / kEncCoefs, 2, args.batch);
+    const dim3 grid(t.N / kEncCoefs * 2 * args.batch);
     hipLaunchKernelGGL(k_encode, grid, dim3(kThreads), enc_smem_bytes(t), stream, t, args.a, args.b,
                        args.planes);
     return hipGetLastError();
@@ -757,7 +836,7 @@ hipError_t launch_forward_columns(const DevTables& t, const BatchArgs& args, hip
 
 hipError_t launch_middle(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     if (!shapes_ok(t, args)) return hipErrorInvalidValue;
-    const dim3 grid(t.N1 / kMidBlocks, args.batch, t.P);
+    const dim3 grid(t.N1 / kMidBlocks * args.batch * t.P);
     hipLaunchKernelGGL(k_middle, grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
     return hipGetLastError();
 }
@@ -769,7 +848,7 @@ hipError_t launch_inverse_columns(const DevTables& t, const BatchArgs& args, hip
 
 hipError_t launch_crt(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     if (!shapes_ok(t, args)) return hipErrorInvalidValue;
-    const dim3 grid(t.N / kCrtCoefs, 1, args.batch);
+    const dim3 grid(t.N / kCrtCoefs * args.batch);
     hipLaunchKernelGGL(k_crt, grid, dim3(kCrtThreads), crt_smem_bytes(t), stream, t, args.planes, args.out);
     return hipGetLastError();
 }
