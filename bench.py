@@ -35,6 +35,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-whole-call", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-qps", type=float, default=1000.0)
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=4.0)
     ap.add_argument("--only-stage", choices=["encode", "fwd_cols", "middle", "inv_cols", "crt", "product"],
                     help="profiling mode: run one stage (or the whole product) --reps times on the batch")
     ap.add_argument("--reps", type=int, default=5)
@@ -165,6 +170,148 @@ def fherma_run_timing():
             "plain_serial_B128_ms": round(B * ms1, 2), "batched_vs_plain": round(B * ms1 / bms, 3),
             "init_s": res[B]["init_s"], "exact": bool(res[B]["exact"] and res[1]["exact"]),
             "reference_rtx_pro_6000": {"B128_ms": 89.648935, "B1_ms": 1.674401, "plain_serial_B128_ms": 291.519151}}
+
+
+def with_edges(torch, x, q: int, fraction: float, seed: int):
+    """Replace `fraction` of the coefficients of x ([..., N, L] int32 limbs) by
+    edge values {0, 1, q-1, floor(q/2), q-floor(q/2)}."""
+    L = x.shape[-1]
+    vals = [0, 1, q - 1, q // 2, q - q // 2]
+    table = torch.tensor([[(v >> (32 * i)) & 0xFFFFFFFF for i in range(L)] for v in vals], dtype=torch.int64,
+                         device=x.device)
+    table = torch.where(table >= (1 << 31), table - (1 << 32), table).to(torch.int32)
+    g = torch.Generator(device=x.device)
+    g.manual_seed(seed)
+    flat = x.reshape(-1, L)
+    mask = torch.rand(flat.shape[0], generator=g, device=x.device) < fraction
+    pick = torch.randint(0, len(vals), (flat.shape[0],), generator=g, device=x.device)
+    flat[mask] = table[pick[mask]]
+    return x
+
+
+UNEVEN = [(32768, 868, 32), (32768, 868, 32), (32768, 868, 32), (65536, 1747, 11), (65536, 1747, 11),
+          (65536, 1747, 10)]
+
+
+def uneven_leg(pkg, torch, steps: int):
+    """One launch per stage over 96 products at N=32,768/W=868 and 32 at
+    N=65,536/W=1,747, three seeded moduli per geometry, 10% edge coefficients;
+    against the same groups launched one after another."""
+    groups, engines = [], []
+    for k, (n, w, cnt) in enumerate(UNEVEN):
+        q = pkg.datagen.modulus(w, 400 + k)
+        e = pkg.Engine(n, w, q, max_batch=cnt)
+        a = with_edges(torch, device_operands(torch, cnt, n, q, 500 + k), q, 0.1, 600 + k)
+        b = with_edges(torch, device_operands(torch, cnt, n, q, 700 + k), q, 0.1, 800 + k)
+        c = torch.empty_like(a)
+        groups.append((e, a, b, c, cnt))
+        engines.append(e)
+    torch.cuda.synchronize()
+    mix = pkg.Mix(groups)
+    stream = torch.cuda.Stream()
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        for _ in range(steps):
+            fn()
+        e.record(stream)
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / steps
+
+    mixed_ms = timed(lambda: mix.run(stream=stream, graph=True))
+    got = [g[3].clone() for g in groups]
+    separate_ms = timed(lambda: [g[0].run_device(g[1], g[2], g[3], g[4], stream=stream, graph=True)
+                                 for g in groups])
+    exact = all(bool(torch.equal(g[3], r)) for g, r in zip(groups, got))
+    one = torch.empty_like(groups[3][1][0])
+    groups[3][0].run_device(groups[3][1][5], groups[3][2][5], one, 1, stream=stream)
+    torch.cuda.synchronize()
+    exact = exact and bool(torch.equal(one, got[3][5]))
+    total = sum(g[4] for g in groups)
+    mix.close()
+    return {"workload": "96 x (N=32768, W=868) + 32 x (N=65536, W=1747), 3 moduli per geometry, "
+                        "10% edge coefficients, one launch per stage (hipGraph)",
+            "products": total, "mixed_ms": round(mixed_ms, 3),
+            "mixed_products_per_s": round(total / mixed_ms * 1e3, 1),
+            "separate_launches_ms": round(separate_ms, 3), "exact": exact}
+
+
+SERVE_GEOMS = [(8192, 220), (16384, 436), (32768, 868), (65536, 1747)]
+
+
+def serving_leg(pkg, qps: float, seconds: float, budget_gib: float):
+    """Poisson arrivals at a fixed rate, N drawn uniformly from SERVE_GEOMS,
+    through the continuous-batching server; every result checked."""
+    import threading
+    rng = np.random.default_rng(2026)
+    pools = []
+    for k, (n, w) in enumerate(SERVE_GEOMS):
+        q = pkg.datagen.modulus(w, 900 + k)
+        e = pkg.Engine(n, w, q, max_batch=1)
+        entries = []
+        for i in range(3):
+            a = pkg.datagen.operands(n, q, 1000 + 10 * k + i)
+            b = pkg.datagen.operands(n, q, 2000 + 10 * k + i)
+            entries.append((a, b, e.run(a, b)))
+        e.close()
+        pools.append((n, w, q, entries))
+    count = max(1, int(qps * seconds))
+    gaps = rng.exponential(1.0 / qps, size=count)
+    kinds = rng.integers(0, len(SERVE_GEOMS), size=count)
+    picks = rng.integers(0, 3, size=count)
+    srv = pkg.Server(hbm_budget_bytes=int(budget_gib * (1 << 30)), max_wave=128)
+    # warm the plan cache (table upload is a one-time cost per geometry)
+    for n, w, q, entries in pools:
+        srv.wait(srv.submit(n, w, q, entries[0][0], entries[0][1]))
+    tickets = [None] * count
+    done_at = [0.0] * count
+    lat = [0.0] * count
+    ok = [False] * count
+    submitted = threading.Semaphore(0)
+
+    def waiter():
+        for i in range(count):
+            submitted.acquire()
+            c, lat[i] = srv.wait(tickets[i])
+            done_at[i] = time.perf_counter()
+            n, w, q, entries = pools[kinds[i]]
+            ok[i] = bool(np.array_equal(c, entries[picks[i]][2]))
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    due = t0
+    for i in range(count):
+        due += gaps[i]
+        now = time.perf_counter()
+        if due > now:
+            time.sleep(due - now)
+        n, w, q, entries = pools[kinds[i]]
+        a, b, _ = entries[picks[i]]
+        tickets[i] = srv.submit(n, w, q, a, b)
+        submitted.release()
+    th.join()
+    span = max(done_at) - t0
+    st = srv.stats()
+    srv.close()
+    lat_ms = np.array(lat) * 1e3
+    per = {}
+    for k, (n, w) in enumerate(SERVE_GEOMS):
+        sel = lat_ms[kinds == k]
+        if sel.size:
+            per[f"N{n}"] = {"requests": int(sel.size), "p50_ms": round(float(np.percentile(sel, 50)), 3),
+                            "p99_ms": round(float(np.percentile(sel, 99)), 3)}
+    return {"workload": "Poisson arrivals, N uniform over {8192, 16384, 32768, 65536} with W {220, 436, 868, "
+                        "1747}, host operands in and results out (whole call per request)",
+            "offered_qps": qps, "requests": count, "hbm_budget_gib": budget_gib,
+            "products_per_s": round(count / span, 1),
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat_ms, 99)), 3), "per_geometry": per,
+            "waves": st["waves"] - len(SERVE_GEOMS), "max_wave": st["max_wave"], "exact": all(ok)}
 
 
 def cpu_baseline(pkg, q, seconds: float):
@@ -326,6 +473,9 @@ def main():
     ntt_gbs = ntt_bytes / (ntt_ms * 1e-3) / 1e9
 
     whole = None if args.no_whole_call else whole_call(eng, a, b, c, torch)
+    uneven = None if args.no_uneven else uneven_leg(pkg, torch, max(3, args.steps // 2))
+    serving = None if args.no_serving else serving_leg(pkg, args.serve_qps, args.serve_seconds,
+                                                       args.serve_budget_gib)
     cpu = None if args.no_cpu else cpu_baseline(pkg, q, args.cpu_seconds)
     line = {
         "metric": "exact products/sec, B=128 N=65,536 W=1,747 (batched vs plain); p50/p99 latency",
@@ -354,6 +504,8 @@ def main():
         "ntt_hbm_frac": round(ntt_gbs / HBM_PEAK_GBS, 4),
         "roofline": roof,
         "whole_call": whole,
+        "uneven": uneven,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -408,10 +408,19 @@ extern "C" int bcrt_set_pipeline(bcrt_engine* e, uint32_t chunk, uint32_t stream
 // ---------------------------------------------------------------------------
 // Mixed batches
 // ---------------------------------------------------------------------------
+// Groups are classed by geometry (N, W); the products of each class are dealt
+// into up to kMixChunks chunks (each group split evenly).  A chunk is one
+// mixed launch per stage over the class's moduli, and chunks run on their own
+// worker streams, so stages of different chunks and classes overlap.
+constexpr uint32_t kMixChunks = 4;
+constexpr uint32_t kMixChunkMin = 8;  // products per chunk before splitting further
+
 struct bcrt_mix {
-    std::vector<bcrt::GroupDesc> groups;
-    bcrt::GroupDesc* d_groups = nullptr;
+    std::vector<std::vector<bcrt::GroupDesc>> chunks;
     hipStream_t stream = nullptr;
+    std::vector<hipStream_t> workers;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::string last_error;
@@ -422,12 +431,34 @@ namespace {
 void release_mix(bcrt_mix* m) {
     if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
-    if (m->d_groups) (void)hipFree(m->d_groups);
+    for (auto ev : m->joins) (void)hipEventDestroy(ev);
+    for (auto st : m->workers) (void)hipStreamDestroy(st);
+    if (m->fork) (void)hipEventDestroy(m->fork);
     if (m->stream) (void)hipStreamDestroy(m->stream);
+    m->joins.clear();
+    m->workers.clear();
+    m->fork = nullptr;
     m->graph_exec = nullptr;
     m->graph = nullptr;
-    m->d_groups = nullptr;
     m->stream = nullptr;
+}
+
+// Enqueue every chunk on `origin` (forked over the worker streams when there
+// are several; also valid inside a stream capture).
+void enqueue_mix(bcrt_mix* m, hipStream_t origin) {
+    if (m->chunks.size() == 1) {
+        check(bcrt::launch_product_groups(m->chunks[0].data(), uint32_t(m->chunks[0].size()), origin),
+              "launch mixed batch");
+        return;
+    }
+    check(hipEventRecord(m->fork, origin), "fork");
+    for (size_t c = 0; c < m->chunks.size(); ++c) {
+        check(hipStreamWaitEvent(m->workers[c], m->fork, 0), "fork wait");
+        check(bcrt::launch_product_groups(m->chunks[c].data(), uint32_t(m->chunks[c].size()), m->workers[c]),
+              "launch mixed chunk");
+        check(hipEventRecord(m->joins[c], m->workers[c]), "join record");
+        check(hipStreamWaitEvent(origin, m->joins[c], 0), "join wait");
+    }
 }
 
 }  // namespace
@@ -441,7 +472,7 @@ extern "C" int bcrt_mix_create(bcrt_mix** out, const bcrt_group* groups, uint32_
     try {
         if (!groups || ngroups == 0 || ngroups > bcrt::kMaxGroups)
             throw std::invalid_argument("ByteCRT mix needs 1.." + std::to_string(bcrt::kMaxGroups) + " groups");
-        uint32_t begin[5] = {0, 0, 0, 0, 0};
+        uint32_t total = 0;
         for (uint32_t k = 0; k < ngroups; ++k) {
             const bcrt_group& g = groups[k];
             if (!g.engine || !g.d_a || !g.d_b || !g.d_c)
@@ -450,20 +481,65 @@ extern "C" int bcrt_mix_create(bcrt_mix** out, const bcrt_group* groups, uint32_
                 if (groups[i].engine == g.engine)
                     throw std::invalid_argument("ByteCRT mix groups must use distinct engines");
             require_batch(g.engine, g.batch);
-            bcrt::GroupDesc d{};
-            d.t = g.engine->tables;
-            d.args = bcrt::BatchArgs{g.d_a, g.d_b, g.d_c, g.engine->planes, g.batch};
-            for (int st = 0; st < 5; ++st) {
-                d.begin[st] = begin[st];
-                begin[st] += bcrt::group_workgroups(d.t, g.batch, st);
-            }
-            m->groups.push_back(d);
+            total += g.batch;
         }
+        // Groups of one geometry (same kernel resource needs) share chunks;
+        // each geometry class is dealt into up to kMixChunks chunks.
+        std::vector<std::vector<uint32_t>> classes;
+        for (uint32_t k = 0; k < ngroups; ++k) {
+            const auto& t = groups[k].engine->tables;
+            bool placed = false;
+            for (auto& cl : classes) {
+                const auto& u = groups[cl[0]].engine->tables;
+                if (u.N == t.N && u.W == t.W) {
+                    cl.push_back(k);
+                    placed = true;
+                    break;
+                }
+            }
+            if (!placed) classes.push_back({k});
+        }
+        (void)total;
+        for (const auto& cl : classes) {
+            uint32_t in_class = 0;
+            for (uint32_t k : cl) in_class += groups[k].batch;
+            const uint32_t nchunks = std::max(1u, std::min(kMixChunks, in_class / kMixChunkMin));
+            for (uint32_t c = 0; c < nchunks; ++c) {
+                std::vector<bcrt::GroupDesc> chunk;
+                uint32_t begin[5] = {0, 0, 0, 0, 0};
+                for (uint32_t k : cl) {
+                    const bcrt_group& g = groups[k];
+                    const uint32_t n = g.batch / nchunks + (c < g.batch % nchunks ? 1 : 0);
+                    if (n == 0) continue;
+                    const uint32_t first = c * (g.batch / nchunks) + std::min(c, g.batch % nchunks);
+                    const auto& t = g.engine->tables;
+                    const size_t io = size_t(t.N) * t.L, pl = size_t(2) * t.P * t.N;
+                    bcrt::GroupDesc d{};
+                    d.t = t;
+                    d.args = bcrt::BatchArgs{g.d_a + first * io, g.d_b + first * io, g.d_c + first * io,
+                                             g.engine->planes + first * pl, n};
+                    for (int st = 0; st < 5; ++st) {
+                        d.begin[st] = begin[st];
+                        begin[st] += bcrt::group_workgroups(d.t, n, st);
+                    }
+                    chunk.push_back(d);
+                }
+                m->chunks.push_back(std::move(chunk));
+            }
+        }
+        const uint32_t nchunks = uint32_t(m->chunks.size());
         check(hipStreamCreate(&m->stream), "create mix stream");
-        check(hipMalloc(reinterpret_cast<void**>(&m->d_groups), sizeof(bcrt::GroupDesc) * ngroups),
-              "allocate group table");
-        check(hipMemcpy(m->d_groups, m->groups.data(), sizeof(bcrt::GroupDesc) * ngroups, hipMemcpyHostToDevice),
-              "upload group table");
+        if (nchunks > 1) {
+            check(hipEventCreateWithFlags(&m->fork, hipEventDisableTiming), "create fork event");
+            for (uint32_t c = 0; c < nchunks; ++c) {
+                hipStream_t st = nullptr;
+                check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "create mix worker stream");
+                m->workers.push_back(st);
+                hipEvent_t ev = nullptr;
+                check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "create join event");
+                m->joins.push_back(ev);
+            }
+        }
     } catch (const std::invalid_argument& x) {
         rc = BCRT_EINVAL;
         m->last_error = x.what();
@@ -485,17 +561,23 @@ extern "C" int bcrt_mix_run(bcrt_mix* m, void* stream, int graph) {
     if (!m) return BCRT_EINVAL;
     try {
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m->stream;
-        const uint32_t n = uint32_t(m->groups.size());
         if (!graph) {
-            check(bcrt::launch_product_groups(m->d_groups, m->groups.data(), n, s), "launch mixed batch");
+            enqueue_mix(m, s);
         } else {
             if (!m->graph_exec) {
                 check(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-                const hipError_t le = bcrt::launch_product_groups(m->d_groups, m->groups.data(), n, m->stream);
+                std::string fail;
+                try {
+                    enqueue_mix(m, m->stream);
+                } catch (const std::exception& x) {
+                    fail = x.what();
+                }
                 hipGraph_t g = nullptr;
                 const hipError_t ce = hipStreamEndCapture(m->stream, &g);
-                if (le != hipSuccess && g) (void)hipGraphDestroy(g);
-                check(le, "launch mixed batch (capture)");
+                if (!fail.empty()) {
+                    if (g) (void)hipGraphDestroy(g);
+                    throw std::runtime_error(fail + " (capture)");
+                }
                 check(ce, "end capture");
                 m->graph = g;
                 check(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0), "instantiate graph");

@@ -2,6 +2,8 @@
 // core/solve.cu:191-291 and native-batched/solve.cu:56-156).
 #include "bytecrt_io.hpp"
 
+#include "bytecrt_copy.hpp"
+
 #include <sys/mman.h>
 
 #include <algorithm>
@@ -29,73 +31,8 @@ void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string("ByteCRT ") + what + ": " + hipGetErrorString(e));
 }
 
-// Fixed pool of host threads; copy() splits one memcpy over all of them and the
-// calling thread (one copy at a time).
-class CopyPool {
-   public:
-    explicit CopyPool(unsigned workers) {
-        for (unsigned i = 0; i < workers; ++i) threads_.emplace_back([this, i] { loop(i); });
-    }
-    ~CopyPool() {
-        {
-            std::lock_guard<std::mutex> g(m_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto& t : threads_) t.join();
-    }
-    void copy(void* dst, const void* src, size_t bytes) {
-        const size_t parts = threads_.size() + 1;
-        if (bytes < (size_t(1) << 20) || parts == 1) {
-            std::memcpy(dst, src, bytes);
-            return;
-        }
-        const size_t step = (bytes / parts + 4095) / 4096 * 4096;
-        auto part = [=](size_t k) {
-            const size_t b0 = std::min(bytes, k * step), b1 = std::min(bytes, (k + 1) * step);
-            if (b1 > b0) std::memcpy(static_cast<char*>(dst) + b0, static_cast<const char*>(src) + b0, b1 - b0);
-        };
-        {
-            std::lock_guard<std::mutex> g(m_);
-            job_ = part;
-            pending_ = threads_.size();
-            ++gen_;
-        }
-        cv_.notify_all();
-        part(0);
-        std::unique_lock<std::mutex> lk(m_);
-        done_cv_.wait(lk, [this] { return pending_ == 0; });
-    }
+}  // namespace
 
-   private:
-    void loop(unsigned i) {
-        uint64_t seen = 0;
-        for (;;) {
-            std::function<void(size_t)> job;
-            {
-                std::unique_lock<std::mutex> lk(m_);
-                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
-                seen = gen_;
-                job = job_;
-            }
-            job(i + 1);
-            std::lock_guard<std::mutex> g(m_);
-            if (--pending_ == 0) done_cv_.notify_one();
-        }
-    }
-    std::vector<std::thread> threads_;
-    std::mutex m_;
-    std::condition_variable cv_, done_cv_;
-    std::function<void(size_t)> job_;
-    size_t pending_ = 0;
-    uint64_t gen_ = 0;
-    bool stop_ = false;
-};
-
-// Asks for transparent huge pages on the 2 MiB-aligned interior of a caller's
-// output array, so first-touch faults of a fresh buffer come 2 MiB at a time.
-// Only page size changes; failure (no THP) is harmless.
 void advise_huge(void* p, size_t bytes) {
     const uintptr_t huge = uintptr_t(2) << 20;
     const uintptr_t b = (uintptr_t(p) + huge - 1) & ~(huge - 1), e = (uintptr_t(p) + bytes) & ~(huge - 1);
@@ -106,8 +43,6 @@ unsigned pool_workers() {
     const unsigned hw = std::max(2u, std::thread::hardware_concurrency());
     return std::min(7u, hw / 2 - 1);
 }
-
-}  // namespace
 
 struct HostIo::Impl {
     struct Job {

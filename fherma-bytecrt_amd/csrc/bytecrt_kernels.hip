@@ -727,66 +727,99 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt(DevTables t, const uint32_t
     crt_wg(t, planes, out, x, z, smem_crt);
 }
 
-// Finds this workgroup's group and its (x, y, z) in that group's grid.
-__device__ __forceinline__ const GroupDesc* locate(const GroupDesc* __restrict__ g, uint32_t ng, int stage,
-                                                   uint32_t* x, uint32_t* y, uint32_t* z) {
+// Finds this workgroup's group and its (x, y, z) in that group's grid, or
+// returns null for the padding workgroups.  Each group's workgroups start at a
+// multiple of 8 (group_workgroups pads the count), so the XCD renumbering is
+// applied inside every group: each XCD gets an eighth of every group's tiles,
+// whatever their cost.
+// (The table is only ever indexed, never addressed, so it stays in the kernel
+// argument segment.)
+__device__ __forceinline__ bool locate(const GroupTable& g, int stage, uint32_t* k_out, uint32_t* x, uint32_t* y,
+                                       uint32_t* z) {
     uint32_t k = 0;
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    while (k + 1 < ng && tile >= g[k + 1].begin[stage]) ++k;
-    const GroupDesc* d = g + k;
+    while (k + 1 < g.n && blockIdx.x >= g.g[k + 1].begin[stage]) ++k;
     uint32_t gx, gy, gz;
-    stage_grid(d->t, d->args.batch, stage, &gx, &gy, &gz);
-    tile_xyz(tile - d->begin[stage], gx, gy, x, y, z);
-    return d;
+    stage_grid(g.g[k].t, g.g[k].args.batch, stage, &gx, &gy, &gz);
+    const uint32_t count = gx * gy * gz;
+    const uint32_t tile = xcd_tile(blockIdx.x - g.g[k].begin[stage], (count + 7) / 8 * 8);
+    *k_out = k;
+    tile_xyz(tile, gx, gy, x, y, z);
+    return tile < count;
 }
 
-__global__ __launch_bounds__(kThreads) void k_encode_groups(const GroupDesc* __restrict__ g, uint32_t ng) {
+// KS / NLOG1 = 0: any mix of shapes (switch per workgroup); otherwise every
+// group of the launch has that shape and the code is specialised for it.
+template <int KS>
+__global__ __launch_bounds__(kThreads) void k_encode_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_enc_g[];
     uint32_t x, y, z;
-    const GroupDesc* d = locate(g, ng, 0, &x, &y, &z);
-    encode_dispatch(d->t, d->args.a, d->args.b, d->args.planes, x, y, z, smem_enc_g);
+    uint32_t k;
+    if (!locate(g, 0, &k, &x, &y, &z)) return;
+    const GroupDesc& d = g.g[k];
+    if constexpr (KS == 0)
+        encode_dispatch(d.t, d.args.a, d.args.b, d.args.planes, x, y, z, smem_enc_g);
+    else
+        encode_wg<KS>(d.t, d.args.a, d.args.b, d.args.planes, x, y, z, smem_enc_g);
 }
 
-__global__ __launch_bounds__(kThreads) void k_fwd_cols_groups(const GroupDesc* __restrict__ g, uint32_t ng) {
+template <int NLOG1>
+__global__ __launch_bounds__(kThreads) void k_fwd_cols_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_fc_g[];
     uint32_t x, y, z;
-    const GroupDesc* d = locate(g, ng, 1, &x, &y, &z);
+    uint32_t k;
+    if (!locate(g, 1, &k, &x, &y, &z)) return;
+    const GroupDesc& d = g.g[k];
     double* s = reinterpret_cast<double*>(smem_fc_g);
-    switch (d->t.n1) {
-        case 4: fwd_cols_wg<4>(d->t, d->args.planes, x, y, z, s); break;
-        case 5: fwd_cols_wg<5>(d->t, d->args.planes, x, y, z, s); break;
-        case 6: fwd_cols_wg<6>(d->t, d->args.planes, x, y, z, s); break;
-        case 7: fwd_cols_wg<7>(d->t, d->args.planes, x, y, z, s); break;
-        default: fwd_cols_wg<8>(d->t, d->args.planes, x, y, z, s); break;
+    if constexpr (NLOG1 != 0) {
+        fwd_cols_wg<NLOG1>(d.t, d.args.planes, x, y, z, s);
+    } else {
+        switch (d.t.n1) {
+            case 4: fwd_cols_wg<4>(d.t, d.args.planes, x, y, z, s); break;
+            case 5: fwd_cols_wg<5>(d.t, d.args.planes, x, y, z, s); break;
+            case 6: fwd_cols_wg<6>(d.t, d.args.planes, x, y, z, s); break;
+            case 7: fwd_cols_wg<7>(d.t, d.args.planes, x, y, z, s); break;
+            default: fwd_cols_wg<8>(d.t, d.args.planes, x, y, z, s); break;
+        }
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_middle_groups(const GroupDesc* __restrict__ g, uint32_t ng) {
+__global__ __launch_bounds__(kThreads) void k_middle_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_mid_g[];
     uint32_t x, y, z;
-    const GroupDesc* d = locate(g, ng, 2, &x, &y, &z);
-    middle_wg(d->t, d->args.planes, x, y, z, reinterpret_cast<double*>(smem_mid_g));
+    uint32_t k;
+    if (!locate(g, 2, &k, &x, &y, &z)) return;
+    const GroupDesc& d = g.g[k];
+    middle_wg(d.t, d.args.planes, x, y, z, reinterpret_cast<double*>(smem_mid_g));
 }
 
-__global__ __launch_bounds__(kThreads) void k_inv_cols_groups(const GroupDesc* __restrict__ g, uint32_t ng) {
+template <int NLOG1>
+__global__ __launch_bounds__(kThreads) void k_inv_cols_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_ic_g[];
     uint32_t x, y, z;
-    const GroupDesc* d = locate(g, ng, 3, &x, &y, &z);
+    uint32_t k;
+    if (!locate(g, 3, &k, &x, &y, &z)) return;
+    const GroupDesc& d = g.g[k];
     double* s = reinterpret_cast<double*>(smem_ic_g);
-    switch (d->t.n1) {
-        case 4: inv_cols_wg<4>(d->t, d->args.planes, x, z, s); break;
-        case 5: inv_cols_wg<5>(d->t, d->args.planes, x, z, s); break;
-        case 6: inv_cols_wg<6>(d->t, d->args.planes, x, z, s); break;
-        case 7: inv_cols_wg<7>(d->t, d->args.planes, x, z, s); break;
-        default: inv_cols_wg<8>(d->t, d->args.planes, x, z, s); break;
+    if constexpr (NLOG1 != 0) {
+        inv_cols_wg<NLOG1>(d.t, d.args.planes, x, z, s);
+    } else {
+        switch (d.t.n1) {
+            case 4: inv_cols_wg<4>(d.t, d.args.planes, x, z, s); break;
+            case 5: inv_cols_wg<5>(d.t, d.args.planes, x, z, s); break;
+            case 6: inv_cols_wg<6>(d.t, d.args.planes, x, z, s); break;
+            case 7: inv_cols_wg<7>(d.t, d.args.planes, x, z, s); break;
+            default: inv_cols_wg<8>(d.t, d.args.planes, x, z, s); break;
+        }
     }
 }
 
-__global__ __launch_bounds__(kCrtThreads) void k_crt_groups(const GroupDesc* __restrict__ g, uint32_t ng) {
+__global__ __launch_bounds__(kCrtThreads) void k_crt_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_crt_g[];
     uint32_t x, y, z;
-    const GroupDesc* d = locate(g, ng, 4, &x, &y, &z);
-    crt_wg(d->t, d->args.planes, d->args.out, x, z, smem_crt_g);
+    uint32_t k;
+    if (!locate(g, 4, &k, &x, &y, &z)) return;
+    const GroupDesc& d = g.g[k];
+    crt_wg(d.t, d.args.planes, d.args.out, x, z, smem_crt_g);
 }
 
 bool shapes_ok(const DevTables& t, const BatchArgs& args) {
@@ -870,16 +903,17 @@ hipError_t launch_mfma_probe(const int8_t* a, const int8_t* b, int32_t* c, hipSt
 uint32_t group_workgroups(const DevTables& t, uint32_t batch, int stage) {
     uint32_t x, y, z;
     stage_grid(t, batch, stage, &x, &y, &z);
-    return x * y * z;
+    return (x * y * z + 7) / 8 * 8;  // padded to whole rounds of the 8 XCDs
 }
 
-hipError_t launch_product_groups(const GroupDesc* d_groups, const GroupDesc* h_groups, uint32_t ngroups,
-                                 hipStream_t stream) {
+hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipStream_t stream) {
     if (ngroups == 0 || ngroups > kMaxGroups) return hipErrorInvalidValue;
+    GroupTable table{};
+    table.n = ngroups;
     size_t enc = 0, crt = 0;
     uint32_t total[5] = {0, 0, 0, 0, 0};
     for (uint32_t k = 0; k < ngroups; ++k) {
-        const GroupDesc& g = h_groups[k];
+        const GroupDesc& g = groups[k];
         if (!shapes_ok(g.t, g.args)) return hipErrorInvalidValue;
         enc = std::max(enc, enc_smem_bytes(g.t));
         crt = std::max(crt, crt_smem_bytes(g.t));
@@ -887,12 +921,46 @@ hipError_t launch_product_groups(const GroupDesc* d_groups, const GroupDesc* h_g
             if (g.begin[st] != total[st]) return hipErrorInvalidValue;
             total[st] += group_workgroups(g.t, g.args.batch, st);
         }
+        table.g[k] = g;
     }
-    hipLaunchKernelGGL(k_encode_groups, dim3(total[0]), dim3(kThreads), enc, stream, d_groups, ngroups);
-    hipLaunchKernelGGL(k_fwd_cols_groups, dim3(total[1]), dim3(kThreads), kExchangeBytes, stream, d_groups, ngroups);
-    hipLaunchKernelGGL(k_middle_groups, dim3(total[2]), dim3(kThreads), kExchangeBytes, stream, d_groups, ngroups);
-    hipLaunchKernelGGL(k_inv_cols_groups, dim3(total[3]), dim3(kThreads), kExchangeBytes, stream, d_groups, ngroups);
-    hipLaunchKernelGGL(k_crt_groups, dim3(total[4]), dim3(kCrtThreads), crt, stream, d_groups, ngroups);
+    // Specialised kernels when every group shares the shape they depend on.
+    bool same_k = true, same_n1 = true;
+    for (uint32_t k = 1; k < ngroups; ++k) {
+        same_k = same_k && groups[k].t.Kenc == groups[0].t.Kenc;
+        same_n1 = same_n1 && groups[k].t.n1 == groups[0].t.n1;
+    }
+    const uint32_t ks = same_k ? groups[0].t.Kenc / 32 : 0, n1 = same_n1 ? groups[0].t.n1 : 0;
+    const dim3 blk(kThreads);
+    switch (ks) {
+        case 1: hipLaunchKernelGGL(k_encode_groups<1>, dim3(total[0]), blk, enc, stream, table); break;
+        case 2: hipLaunchKernelGGL(k_encode_groups<2>, dim3(total[0]), blk, enc, stream, table); break;
+        case 3: hipLaunchKernelGGL(k_encode_groups<3>, dim3(total[0]), blk, enc, stream, table); break;
+        case 4: hipLaunchKernelGGL(k_encode_groups<4>, dim3(total[0]), blk, enc, stream, table); break;
+        case 5: hipLaunchKernelGGL(k_encode_groups<5>, dim3(total[0]), blk, enc, stream, table); break;
+        case 6: hipLaunchKernelGGL(k_encode_groups<6>, dim3(total[0]), blk, enc, stream, table); break;
+        case 7: hipLaunchKernelGGL(k_encode_groups<7>, dim3(total[0]), blk, enc, stream, table); break;
+        case 8: hipLaunchKernelGGL(k_encode_groups<8>, dim3(total[0]), blk, enc, stream, table); break;
+        default: hipLaunchKernelGGL(k_encode_groups<0>, dim3(total[0]), blk, enc, stream, table); break;
+    }
+    const size_t xb = kExchangeBytes;
+    switch (n1) {
+        case 4: hipLaunchKernelGGL(k_fwd_cols_groups<4>, dim3(total[1]), blk, xb, stream, table); break;
+        case 5: hipLaunchKernelGGL(k_fwd_cols_groups<5>, dim3(total[1]), blk, xb, stream, table); break;
+        case 6: hipLaunchKernelGGL(k_fwd_cols_groups<6>, dim3(total[1]), blk, xb, stream, table); break;
+        case 7: hipLaunchKernelGGL(k_fwd_cols_groups<7>, dim3(total[1]), blk, xb, stream, table); break;
+        case 8: hipLaunchKernelGGL(k_fwd_cols_groups<8>, dim3(total[1]), blk, xb, stream, table); break;
+        default: hipLaunchKernelGGL(k_fwd_cols_groups<0>, dim3(total[1]), blk, xb, stream, table); break;
+    }
+    hipLaunchKernelGGL(k_middle_groups, dim3(total[2]), blk, xb, stream, table);
+    switch (n1) {
+        case 4: hipLaunchKernelGGL(k_inv_cols_groups<4>, dim3(total[3]), blk, xb, stream, table); break;
+        case 5: hipLaunchKernelGGL(k_inv_cols_groups<5>, dim3(total[3]), blk, xb, stream, table); break;
+        case 6: hipLaunchKernelGGL(k_inv_cols_groups<6>, dim3(total[3]), blk, xb, stream, table); break;
+        case 7: hipLaunchKernelGGL(k_inv_cols_groups<7>, dim3(total[3]), blk, xb, stream, table); break;
+        case 8: hipLaunchKernelGGL(k_inv_cols_groups<8>, dim3(total[3]), blk, xb, stream, table); break;
+        default: hipLaunchKernelGGL(k_inv_cols_groups<0>, dim3(total[3]), blk, xb, stream, table); break;
+    }
+    hipLaunchKernelGGL(k_crt_groups, dim3(total[4]), dim3(kCrtThreads), crt, stream, table);
     return hipGetLastError();
 }
 

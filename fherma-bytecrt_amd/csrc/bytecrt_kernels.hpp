@@ -41,20 +41,28 @@ struct BatchArgs {
 
 // One group of a mixed launch: a plan and a batch of products under it.
 // begin[s] is the group's first workgroup in stage s (prefix sums over the
-// groups, in order; group_workgroups gives each group's count).
-constexpr uint32_t kMaxGroups = 64;
+// groups, in order; group_workgroups gives each group's count).  The table
+// travels by value in the kernel arguments (16 groups fit the 4 KiB limit),
+// so its pointers are known to be global and nothing is uploaded per launch.
+constexpr uint32_t kMaxGroups = 16;
 struct GroupDesc {
     DevTables t;
     BatchArgs args;
     uint32_t begin[5];
 };
+struct GroupTable {
+    GroupDesc g[kMaxGroups];
+    uint32_t n;
+};
+static_assert(sizeof(GroupTable) <= 4096, "kernel argument limit");
 
+// Workgroups of stage `stage` for `batch` products of plan t in a mixed
+// launch (padded to a multiple of 8; the padding workgroups exit at once).
 uint32_t group_workgroups(const DevTables& t, uint32_t batch, int stage);
 
-// Enqueue the five stages for all groups: one launch per stage covering every
-// product of every group.  d_groups is the device copy of h_groups.
-hipError_t launch_product_groups(const GroupDesc* d_groups, const GroupDesc* h_groups, uint32_t ngroups,
-                                 hipStream_t stream);
+// Enqueue the five stages for groups[0..ngroups): one launch per stage
+// covering every product of every group.
+hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipStream_t stream);
 
 // Enqueue the whole product pipeline for a batch on `stream` (no host sync).
 hipError_t launch_product(const DevTables& t, const BatchArgs& args, hipStream_t stream);

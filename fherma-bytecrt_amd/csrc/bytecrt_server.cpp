@@ -1,9 +1,15 @@
 // Continuous-batching product server: requests of mixed geometry arrive at any
-// time, are admitted into "waves" while HBM budget and in-flight slots allow,
-// and each wave runs as one mixed launch per stage (bcrt group kernels) on one
-// of two compute streams, so a wave admitted while another is on the GPU
-// overlaps with it.  Plan tables stay resident per (N, W, q); per-request
-// device memory (operands, result, residue planes) comes from one arena.
+// time, are admitted into "waves" while the HBM budget, pinned staging and
+// in-flight slots allow, and each wave runs as one mixed launch per stage
+// (group kernels) on one of two compute streams, so a wave admitted while
+// another is on the GPU overlaps with it.  Plan tables stay resident per
+// (N, W, q); per-request device memory (operands, result, residue planes)
+// comes from one arena used as a FIFO ring.
+//
+// Two host threads: the admit thread forms waves, packs operands into pinned
+// staging (copy pool) and enqueues copies and kernels; the retire thread
+// waits for the oldest wave, unpacks results (its own copy pool) and releases
+// the wave's ring space.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -21,6 +27,7 @@
 #include <vector>
 
 #include "../../include/bytecrt.h"
+#include "bytecrt_copy.hpp"
 #include "bytecrt_kernels.hpp"
 
 namespace bcrt {
@@ -36,7 +43,7 @@ void hcheck(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string("ByteCRT server ") + what + ": " + hipGetErrorString(e));
 }
 
-constexpr uint32_t kInflight = 4;   // waves on the device at once
+constexpr uint32_t kInflight = 4;  // waves on the device at once
 constexpr size_t kAlign = 256;
 
 struct Plan {
@@ -58,66 +65,39 @@ struct Request {
     std::string error;
 };
 
-// A region of each of the three FIFO rings: device arena, pinned inputs,
-// pinned outputs.
+// The three FIFO rings a wave takes space from.
 enum Ring { kArena = 0, kPinIn = 1, kPinOut = 2 };
 
 struct Wave {
     std::vector<Request*> reqs;
-    std::vector<bcrt::GroupDesc> groups;  // host copy, alive until the upload completes
+    std::vector<bcrt::GroupDesc> groups;
     size_t start[3] = {0, 0, 0}, end[3] = {0, 0, 0};
-    uint32_t slot = 0;
     hipEvent_t inputs_ready = nullptr;
     hipEvent_t done = nullptr;
+    bool enqueued = false;  // set under the lock once enqueue() returned
+    std::string error;      // enqueue() failed
 };
 
 size_t round_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
-}  // namespace
+size_t footprint(const Plan* p, int k) {
+    switch (k) {
+        case kArena: return round_up((p->io_words * 3 + p->plane_words) * sizeof(uint32_t));
+        case kPinIn: return round_up(p->io_words * 2 * sizeof(uint32_t));
+        default: return round_up(p->io_words * sizeof(uint32_t));
+    }
+}
 
-struct bcrt_server {
-    size_t budget = 0;      // device arena bytes
-    size_t cap[3] = {0, 0, 0};
-    uint32_t max_wave = 0;
-    uint8_t* arena = nullptr;
-    uint8_t* pin_in = nullptr;   // pinned staging of operands (cap[kPinIn] bytes)
-    uint8_t* pin_out = nullptr;  // pinned staging of results (cap[kPinOut] bytes)
-    bcrt::GroupDesc* d_groups = nullptr;  // [kInflight][kMaxGroups]
-    hipStream_t copy_in = nullptr;
-    hipStream_t compute[2] = {nullptr, nullptr};
-    uint32_t next_compute = 0;
-    std::mutex m;
-    std::condition_variable cv, done_cv;
-    std::deque<Request*> pending;
-    std::deque<Wave> inflight;
-    std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
-    std::map<std::string, std::unique_ptr<Plan>> plans;
-    std::mutex plan_m;
-    uint64_t next_id = 1;
-    bool stop = false;
-    bool slot_busy[kInflight] = {false, false, false, false};
-    std::string fatal;
-    std::thread worker;
-    // statistics
-    uint64_t completed = 0, waves = 0, max_wave_seen = 0;
-    std::string last_error;
-
-    void run();
-    bool admit(std::unique_lock<std::mutex>& lk);
-    void retire();
-};
-
-namespace {
-
-// Start of a free region of `size` bytes in ring `k` (capacity cap), given the
-// in-flight waves in admission order.
-bool ring_alloc(const std::deque<Wave>& inflight, int k, size_t cap, size_t size, size_t* start) {
+// Start of a free region of `size` bytes in ring `k` of capacity `cap`, given
+// the in-flight waves in admission order.
+bool ring_alloc(const std::deque<std::unique_ptr<Wave>>& inflight, int k, size_t cap, size_t size,
+                size_t* start) {
     if (size > cap) return false;
     if (inflight.empty()) {
         *start = 0;
         return true;
     }
-    const size_t first = inflight.front().start[k], last = inflight.back().end[k];
+    const size_t first = inflight.front()->start[k], last = inflight.back()->end[k];
     if (last > first) {  // not wrapped: free space after `last` and before `first`
         if (cap - last >= size) {
             *start = last;
@@ -136,27 +116,43 @@ bool ring_alloc(const std::deque<Wave>& inflight, int k, size_t cap, size_t size
     return false;
 }
 
-size_t footprint(const Plan* p, int k) {
-    switch (k) {
-        case kArena: return round_up((p->io_words * 3 + p->plane_words) * sizeof(uint32_t));
-        case kPinIn: return round_up(p->io_words * 2 * sizeof(uint32_t));
-        default: return round_up(p->io_words * sizeof(uint32_t));
-    }
-}
-
 }  // namespace
 
-bool bcrt_server::admit(std::unique_lock<std::mutex>& lk) {
-    if (pending.empty()) return false;
-    uint32_t slot = kInflight;
-    for (uint32_t i = 0; i < kInflight; ++i)
-        if (!slot_busy[i]) {
-            slot = i;
-            break;
-        }
-    if (slot == kInflight) return false;
-    // Take pending requests in arrival order while all three rings have room;
-    // group them by plan.
+struct bcrt_server {
+    size_t cap[3] = {0, 0, 0};  // arena (device), pinned inputs, pinned outputs
+    uint32_t max_wave = 0;
+    uint8_t* arena = nullptr;
+    uint8_t* pin_in = nullptr;
+    uint8_t* pin_out = nullptr;
+    hipStream_t copy_in = nullptr;
+    hipStream_t compute[2] = {nullptr, nullptr};
+    uint32_t next_compute = 0;
+    std::unique_ptr<bcrt::CopyPool> pack, unpack;
+    std::mutex m;
+    std::condition_variable admit_cv, retire_cv, done_cv;
+    std::deque<Request*> pending;
+    std::deque<std::unique_ptr<Wave>> inflight;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
+    std::map<std::string, std::unique_ptr<Plan>> plans;
+    std::mutex plan_m;
+    uint64_t next_id = 1;
+    bool stop = false, admit_done = false;
+    std::string fatal;
+    std::thread admitter, retirer;
+    uint64_t completed = 0, waves = 0, max_wave_seen = 0;
+    std::string last_error;
+
+    void admit_loop();
+    void retire_loop();
+    std::unique_ptr<Wave> form_wave();
+    void enqueue(Wave& w);
+};
+
+// Takes pending requests in arrival order while all three rings have room and
+// fewer than kInflight waves are on the device; groups them by plan.  Called
+// with the lock held.
+std::unique_ptr<Wave> bcrt_server::form_wave() {
+    if (pending.empty() || inflight.size() >= kInflight) return nullptr;
     std::vector<Plan*> order;
     std::map<Plan*, std::vector<Request*>> by_plan;
     size_t need[3] = {0, 0, 0};
@@ -175,15 +171,14 @@ bool bcrt_server::admit(std::unique_lock<std::mutex>& lk) {
         by_plan[r->plan].push_back(r);
         ++taken;
     }
-    if (taken == 0) return false;
-    Wave w;
+    if (taken == 0) return nullptr;
+    auto w = std::make_unique<Wave>();
     for (int k = 0; k < 3; ++k) {
-        ring_alloc(inflight, k, cap[k], need[k], &w.start[k]);
-        w.end[k] = w.start[k] + need[k];
+        ring_alloc(inflight, k, cap[k], need[k], &w->start[k]);
+        w->end[k] = w->start[k] + need[k];
     }
     pending.erase(pending.begin(), pending.begin() + taken);
-    w.slot = slot;
-    size_t off = w.start[kArena];
+    size_t off = w->start[kArena];
     uint32_t begin[5] = {0, 0, 0, 0, 0};
     for (Plan* p : order) {
         const auto& rs = by_plan[p];
@@ -201,113 +196,116 @@ bool bcrt_server::admit(std::unique_lock<std::mutex>& lk) {
             d.begin[st] = begin[st];
             begin[st] += bcrt::group_workgroups(d.t, n, st);
         }
-        w.groups.push_back(d);
-        for (uint32_t i = 0; i < n; ++i) w.reqs.push_back(rs[i]);
+        w->groups.push_back(d);
+        for (uint32_t i = 0; i < n; ++i) w->reqs.push_back(rs[i]);
     }
-    slot_busy[slot] = true;
-    lk.unlock();
-    // Device work for the wave, without the lock: pack operands into pinned
-    // staging, copy in on the copy stream, one mixed launch per stage on a
-    // compute stream, copy results to pinned staging.
-    try {
-        hipStream_t cs = compute[next_compute++ & 1];
-        bcrt::GroupDesc* table = d_groups + size_t(slot) * bcrt::kMaxGroups;
-        hcheck(hipMemcpyAsync(table, w.groups.data(), sizeof(bcrt::GroupDesc) * w.groups.size(),
-                              hipMemcpyHostToDevice, copy_in),
-               "upload group table");
-        size_t in_off = w.start[kPinIn], ri = 0;
-        for (const auto& g : w.groups) {
-            const size_t bytes = size_t(g.t.N) * g.t.L * sizeof(uint32_t);
-            for (uint32_t i = 0; i < g.args.batch; ++i, ++ri) {
-                Request* r = w.reqs[ri];
-                uint8_t* h = pin_in + in_off;
-                std::memcpy(h, r->a, bytes);
-                std::memcpy(h + bytes, r->b, bytes);
-                hcheck(hipMemcpyAsync(const_cast<uint32_t*>(g.args.a) + size_t(i) * g.t.N * g.t.L, h, bytes,
-                                      hipMemcpyHostToDevice, copy_in),
-                       "copy a");
-                hcheck(hipMemcpyAsync(const_cast<uint32_t*>(g.args.b) + size_t(i) * g.t.N * g.t.L, h + bytes,
-                                      bytes, hipMemcpyHostToDevice, copy_in),
-                       "copy b");
-                in_off += round_up(2 * bytes);
-            }
-        }
-        hcheck(hipEventCreateWithFlags(&w.inputs_ready, hipEventDisableTiming), "wave event");
-        hcheck(hipEventRecord(w.inputs_ready, copy_in), "record inputs");
-        hcheck(hipStreamWaitEvent(cs, w.inputs_ready, 0), "compute waits inputs");
-        hcheck(bcrt::launch_product_groups(table, w.groups.data(), uint32_t(w.groups.size()), cs), "launch wave");
-        size_t out_off = w.start[kPinOut];
-        for (const auto& g : w.groups) {
-            const size_t bytes = size_t(g.t.N) * g.t.L * sizeof(uint32_t);
-            for (uint32_t i = 0; i < g.args.batch; ++i) {
-                hcheck(hipMemcpyAsync(pin_out + out_off, g.args.out + size_t(i) * g.t.N * g.t.L, bytes,
-                                      hipMemcpyDeviceToHost, cs),
-                       "copy c");
-                out_off += round_up(bytes);
-            }
-        }
-        hcheck(hipEventCreateWithFlags(&w.done, hipEventDisableTiming), "wave event");
-        hcheck(hipEventRecord(w.done, cs), "record wave");
-    } catch (const std::exception& x) {
-        lk.lock();
-        for (Request* r : w.reqs) r->error = x.what(), r->done = true;
-        if (w.inputs_ready) (void)hipEventDestroy(w.inputs_ready);
-        slot_busy[slot] = false;
-        done_cv.notify_all();
-        throw;
-    }
-    lk.lock();
-    ++waves;
-    max_wave_seen = std::max<uint64_t>(max_wave_seen, w.reqs.size());
-    inflight.push_back(std::move(w));
-    return true;
+    return w;
 }
 
-void bcrt_server::retire() {
-    // Waves retire in admission order (the arena is a FIFO ring).
-    while (!inflight.empty()) {
-        Wave& w = inflight.front();
-        const hipError_t q = hipEventQuery(w.done);
-        if (q == hipErrorNotReady) break;
-        size_t out_off = w.start[kPinOut], ri = 0;
-        for (const auto& g : w.groups) {
-            const size_t bytes = size_t(g.t.N) * g.t.L * sizeof(uint32_t);
-            for (uint32_t i = 0; i < g.args.batch; ++i, ++ri) {
-                if (q == hipSuccess) std::memcpy(w.reqs[ri]->c, pin_out + out_off, bytes);
-                out_off += round_up(bytes);
+// Packs the wave's operands into pinned staging and enqueues its copies and
+// kernels (called without the lock).
+void bcrt_server::enqueue(Wave& w) {
+    hipStream_t cs = compute[next_compute++ & 1];
+    size_t in_off = w.start[kPinIn], ri = 0;
+    for (const auto& g : w.groups) {
+        const size_t words = size_t(g.t.N) * g.t.L, bytes = words * sizeof(uint32_t);
+        for (uint32_t i = 0; i < g.args.batch; ++i, ++ri) {
+            const Request* r = w.reqs[ri];
+            uint8_t* h = pin_in + in_off;
+            pack->copy(h, r->a, bytes);
+            pack->copy(h + bytes, r->b, bytes);
+            hcheck(hipMemcpyAsync(const_cast<uint32_t*>(g.args.a) + i * words, h, bytes, hipMemcpyHostToDevice,
+                                  copy_in),
+                   "copy a");
+            hcheck(hipMemcpyAsync(const_cast<uint32_t*>(g.args.b) + i * words, h + bytes, bytes,
+                                  hipMemcpyHostToDevice, copy_in),
+                   "copy b");
+            in_off += round_up(2 * bytes);
+        }
+    }
+    hcheck(hipEventCreateWithFlags(&w.inputs_ready, hipEventDisableTiming), "wave event");
+    hcheck(hipEventRecord(w.inputs_ready, copy_in), "record inputs");
+    hcheck(hipStreamWaitEvent(cs, w.inputs_ready, 0), "compute waits inputs");
+    hcheck(bcrt::launch_product_groups(w.groups.data(), uint32_t(w.groups.size()), cs), "launch wave");
+    size_t out_off = w.start[kPinOut];
+    for (const auto& g : w.groups) {
+        const size_t words = size_t(g.t.N) * g.t.L, bytes = words * sizeof(uint32_t);
+        for (uint32_t i = 0; i < g.args.batch; ++i) {
+            hcheck(hipMemcpyAsync(pin_out + out_off, g.args.out + i * words, bytes, hipMemcpyDeviceToHost, cs),
+                   "copy c");
+            out_off += round_up(bytes);
+        }
+    }
+    hcheck(hipEventCreateWithFlags(&w.done, hipEventDisableTiming), "wave event");
+    hcheck(hipEventRecord(w.done, cs), "record wave");
+}
+
+void bcrt_server::admit_loop() {
+    std::unique_lock<std::mutex> lk(m);
+    for (;;) {
+        std::unique_ptr<Wave> w;
+        admit_cv.wait(lk, [&] { return (w = form_wave()) != nullptr || (stop && pending.empty()); });
+        if (!w) break;
+        Wave* raw = w.get();
+        // Published before the device work is enqueued, so the next wave's ring
+        // space is computed after this one's; the retire thread waits for
+        // `enqueued` before touching it.
+        inflight.push_back(std::move(w));
+        ++waves;
+        max_wave_seen = std::max<uint64_t>(max_wave_seen, raw->reqs.size());
+        lk.unlock();
+        std::string err;
+        try {
+            enqueue(*raw);
+        } catch (const std::exception& x) {
+            err = x.what();
+        }
+        lk.lock();
+        raw->error = err;
+        raw->enqueued = true;
+        retire_cv.notify_one();
+    }
+    admit_done = true;
+    retire_cv.notify_one();
+}
+
+void bcrt_server::retire_loop() {
+    std::unique_lock<std::mutex> lk(m);
+    for (;;) {
+        retire_cv.wait(lk, [&] { return (!inflight.empty() && inflight.front()->enqueued) ||
+                                        (admit_done && inflight.empty()); });
+        if (inflight.empty()) break;
+        Wave& w = *inflight.front();
+        std::string err = w.error;
+        lk.unlock();
+        if (err.empty()) {
+            const hipError_t q = hipEventSynchronize(w.done);
+            if (q != hipSuccess) err = std::string("ByteCRT server wave failed: ") + hipGetErrorString(q);
+        }
+        if (err.empty()) {
+            size_t out_off = w.start[kPinOut], ri = 0;
+            for (const auto& g : w.groups) {
+                const size_t bytes = size_t(g.t.N) * g.t.L * sizeof(uint32_t);
+                for (uint32_t i = 0; i < g.args.batch; ++i, ++ri) {
+                    unpack->copy(w.reqs[ri]->c, pin_out + out_off, bytes);
+                    out_off += round_up(bytes);
+                }
             }
         }
         const auto now = Clock::now();
+        if (w.done) (void)hipEventDestroy(w.done);
+        if (w.inputs_ready) (void)hipEventDestroy(w.inputs_ready);
+        lk.lock();
         for (Request* r : w.reqs) {
             r->latency = std::chrono::duration<double>(now - r->submitted).count();
-            if (q != hipSuccess) r->error = std::string("ByteCRT server wave failed: ") + hipGetErrorString(q);
+            r->error = err;
             r->done = true;
             ++completed;
         }
-        (void)hipEventDestroy(w.done);
-        (void)hipEventDestroy(w.inputs_ready);
-        slot_busy[w.slot] = false;
+        if (!err.empty() && fatal.empty()) fatal = err;
         inflight.pop_front();
         done_cv.notify_all();
-    }
-}
-
-void bcrt_server::run() {
-    std::unique_lock<std::mutex> lk(m);
-    while (!stop || !pending.empty() || !inflight.empty()) {
-        try {
-            retire();
-            bool progressed = admit(lk);
-            retire();
-            if (!progressed) cv.wait_for(lk, std::chrono::microseconds(50));
-        } catch (const std::exception& x) {
-            if (!lk.owns_lock()) lk.lock();
-            fatal = x.what();
-            for (Request* r : pending) r->error = fatal, r->done = true;
-            pending.clear();
-            done_cv.notify_all();
-            if (stop) break;
-        }
+        admit_cv.notify_one();
     }
 }
 
@@ -320,36 +318,34 @@ int bcrt_server_create(bcrt_server** out, size_t hbm_budget_bytes, uint32_t max_
     try {
         if (hbm_budget_bytes < (size_t(1) << 20) || max_wave == 0)
             throw std::invalid_argument("ByteCRT server needs a budget >= 1 MiB and max_wave >= 1");
-        s->budget = hbm_budget_bytes / kAlign * kAlign;
-        s->cap[kArena] = s->budget;
+        s->cap[kArena] = hbm_budget_bytes / kAlign * kAlign;
         // Operands are at most half and results a quarter of a product's
         // device footprint (its residue planes outweigh the operands).
-        s->cap[kPinIn] = s->budget / 2 / kAlign * kAlign;
-        s->cap[kPinOut] = s->budget / 4 / kAlign * kAlign;
+        s->cap[kPinIn] = s->cap[kArena] / 2 / kAlign * kAlign;
+        s->cap[kPinOut] = s->cap[kArena] / 4 / kAlign * kAlign;
         s->max_wave = max_wave;
-        hcheck(hipMalloc(reinterpret_cast<void**>(&s->arena), s->budget), "allocate arena");
+        hcheck(hipMalloc(reinterpret_cast<void**>(&s->arena), s->cap[kArena]), "allocate arena");
         hcheck(hipHostMalloc(reinterpret_cast<void**>(&s->pin_in), s->cap[kPinIn], hipHostMallocDefault),
                "allocate pinned inputs");
         hcheck(hipHostMalloc(reinterpret_cast<void**>(&s->pin_out), s->cap[kPinOut], hipHostMallocDefault),
                "allocate pinned outputs");
-        hcheck(hipMalloc(reinterpret_cast<void**>(&s->d_groups),
-                         sizeof(bcrt::GroupDesc) * bcrt::kMaxGroups * kInflight),
-               "allocate group tables");
         hcheck(hipStreamCreateWithFlags(&s->copy_in, hipStreamNonBlocking), "create stream");
         for (auto& c : s->compute) hcheck(hipStreamCreateWithFlags(&c, hipStreamNonBlocking), "create stream");
+        s->pack = std::make_unique<bcrt::CopyPool>(bcrt::pool_workers());
+        s->unpack = std::make_unique<bcrt::CopyPool>(bcrt::pool_workers());
     } catch (const std::exception& x) {
         if (err && errlen) std::snprintf(err, errlen, "%s", x.what());
         if (s->arena) (void)hipFree(s->arena);
         if (s->pin_in) (void)hipHostFree(s->pin_in);
         if (s->pin_out) (void)hipHostFree(s->pin_out);
-        if (s->d_groups) (void)hipFree(s->d_groups);
         for (auto c : s->compute)
             if (c) (void)hipStreamDestroy(c);
         if (s->copy_in) (void)hipStreamDestroy(s->copy_in);
         return dynamic_cast<const std::invalid_argument*>(&x) ? BCRT_EINVAL : BCRT_ERUNTIME;
     }
     bcrt_server* raw = s.release();
-    raw->worker = std::thread([raw] { raw->run(); });
+    raw->admitter = std::thread([raw] { raw->admit_loop(); });
+    raw->retirer = std::thread([raw] { raw->retire_loop(); });
     *out = raw;
     return BCRT_OK;
 }
@@ -379,12 +375,13 @@ int bcrt_server_submit(bcrt_server* s, uint32_t N, uint32_t W, uint32_t L, const
         }
         plan = it->second.get();
     }
-    const size_t need = footprint(plan, kArena);
     std::lock_guard<std::mutex> g(s->m);
-    if (need > s->budget || footprint(plan, kPinIn) > s->cap[kPinIn] || footprint(plan, kPinOut) > s->cap[kPinOut]) {
-        s->last_error = "ByteCRT server: one product needs " + std::to_string(need) + " bytes, above the budget";
-        return BCRT_EINVAL;
-    }
+    for (int k = 0; k < 3; ++k)
+        if (footprint(plan, k) > s->cap[k]) {
+            s->last_error = "ByteCRT server: one product needs " + std::to_string(footprint(plan, kArena)) +
+                            " bytes of HBM, above the budget";
+            return BCRT_EINVAL;
+        }
     if (!s->fatal.empty()) {
         s->last_error = s->fatal;
         return BCRT_ERUNTIME;
@@ -399,7 +396,7 @@ int bcrt_server_submit(bcrt_server* s, uint32_t N, uint32_t W, uint32_t L, const
     s->pending.push_back(r.get());
     *ticket = r->id;
     s->requests.emplace(r->id, std::move(r));
-    s->cv.notify_one();
+    s->admit_cv.notify_one();
     return BCRT_OK;
 }
 
@@ -422,11 +419,16 @@ int bcrt_server_wait(bcrt_server* s, uint64_t ticket, double* latency_s) {
 
 int bcrt_server_stats(bcrt_server* s, uint64_t* out) {
     if (!s || !out) return BCRT_EINVAL;
+    size_t nplans;
+    {
+        std::lock_guard<std::mutex> pg(s->plan_m);
+        nplans = s->plans.size();
+    }
     std::lock_guard<std::mutex> g(s->m);
     out[0] = s->completed;
     out[1] = s->waves;
     out[2] = s->max_wave_seen;
-    out[3] = s->plans.size();
+    out[3] = nplans;
     return BCRT_OK;
 }
 
@@ -438,15 +440,15 @@ void bcrt_server_destroy(bcrt_server* s) {
         std::lock_guard<std::mutex> g(s->m);
         s->stop = true;
     }
-    s->cv.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    s->admit_cv.notify_all();
+    if (s->admitter.joinable()) s->admitter.join();
+    if (s->retirer.joinable()) s->retirer.join();
     for (auto c : s->compute)
         if (c) (void)hipStreamSynchronize(c), (void)hipStreamDestroy(c);
     if (s->copy_in) (void)hipStreamDestroy(s->copy_in);
     if (s->arena) (void)hipFree(s->arena);
     if (s->pin_in) (void)hipHostFree(s->pin_in);
     if (s->pin_out) (void)hipHostFree(s->pin_out);
-    if (s->d_groups) (void)hipFree(s->d_groups);
     delete s;
 }
 

@@ -85,7 +85,8 @@ int bcrt_set_pipeline(bcrt_engine* e, uint32_t chunk, uint32_t streams);
 /* ---- Mixed batches (products of different N, W, q in one launch per stage) ----
  * No reference counterpart: the reference runs one fixed geometry per process
  * (native-batched/solve.cu:163-164).  Each group names an engine (its plan
- * and workspace; engines must be distinct) and a device batch for it.  The
+ * and workspace; engines must be distinct; at most 16 groups) and a device
+ * batch for it.  The
  * five stages each run as ONE launch whose workgroups cover every product of
  * every group. */
 typedef struct bcrt_group {

@@ -61,6 +61,37 @@ def test_mix_uneven_batches(pkg, orc, gu):
     mix.close()
 
 
+def test_mix_chunked_over_streams(pkg, orc, gu):
+    """26 products: the mix is dealt into three chunks on worker streams (each
+    group split across chunks); eager and graph replays equal each product
+    computed alone."""
+    spec = [(8192, 220, 12), (4096, 100, 9), (16384, 436, 5)]
+    groups, host = [], []
+    for k, (N, W, B) in enumerate(spec):
+        q = pkg.datagen.modulus(W, 300 + k)
+        e = pkg.Engine(N, W, q, max_batch=B)
+        a = np.stack([pkg.datagen.operands(N, q, 3000 + 50 * k + i, edge_fraction=0.1) for i in range(B)])
+        b = np.stack([pkg.datagen.operands(N, q, 4000 + 50 * k + i) for i in range(B)])
+        out = gu.empty_dev(B * N * e.L)
+        groups.append((e, gu.to_dev(a), gu.to_dev(b), out, B))
+        host.append((e, q, a, b, out))
+    mix = pkg.Mix(groups)
+    want = []
+    for e, q, a, b, out in host:
+        want.append(np.stack([alone(pkg, gu, e, a[i], b[i]) for i in range(a.shape[0])]))
+    for graph in (False, True, True):
+        for *_, out in host:
+            out.zero_()
+        gu.sync()
+        mix.run(graph=graph)
+        gu.sync()
+        for (e, *_r, out), w in zip(host, want):
+            np.testing.assert_array_equal(gu.from_dev(out).reshape(-1, e.N, e.L), w, err_msg=f"N={e.N}")
+    e, q, a, b, _ = host[2]
+    np.testing.assert_array_equal(want[2][4], orc.Plan(e.N, e.W, q).product(a[4], b[4]))
+    mix.close()
+
+
 def test_mix_rejects_bad_groups(pkg, gu):
     q = pkg.datagen.modulus(100, 1)
     e = pkg.Engine(4096, 100, q, max_batch=1)
