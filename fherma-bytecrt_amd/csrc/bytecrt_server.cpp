@@ -29,6 +29,7 @@
 #include "../../include/bytecrt.h"
 #include "bytecrt_copy.hpp"
 #include "bytecrt_kernels.hpp"
+#include "bytecrt_sched.hpp"
 
 namespace bcrt {
 // Tables and geometry of an engine (bytecrt_engine.cpp).
@@ -65,7 +66,7 @@ struct Request {
     std::string error;
 };
 
-// The three FIFO rings a wave takes space from.
+// The three FIFO rings a wave takes space from (WaveScheduler's rings).
 enum Ring { kArena = 0, kPinIn = 1, kPinOut = 2 };
 
 struct Wave {
@@ -88,38 +89,10 @@ size_t footprint(const Plan* p, int k) {
     }
 }
 
-// Start of a free region of `size` bytes in ring `k` of capacity `cap`, given
-// the in-flight waves in admission order.
-bool ring_alloc(const std::deque<std::unique_ptr<Wave>>& inflight, int k, size_t cap, size_t size,
-                size_t* start) {
-    if (size > cap) return false;
-    if (inflight.empty()) {
-        *start = 0;
-        return true;
-    }
-    const size_t first = inflight.front()->start[k], last = inflight.back()->end[k];
-    if (last > first) {  // not wrapped: free space after `last` and before `first`
-        if (cap - last >= size) {
-            *start = last;
-            return true;
-        }
-        if (first >= size) {
-            *start = 0;
-            return true;
-        }
-        return false;
-    }
-    if (first - last >= size) {  // wrapped (last <= first): free space between them
-        *start = last;
-        return true;
-    }
-    return false;
-}
-
 }  // namespace
 
 struct bcrt_server {
-    size_t cap[3] = {0, 0, 0};  // arena (device), pinned inputs, pinned outputs
+    size_t cap[bcrt::kRings] = {0, 0, 0};  // arena (device), pinned inputs, pinned outputs
     uint32_t max_wave = 0;
     uint8_t* arena = nullptr;
     uint8_t* pin_in = nullptr;
@@ -130,8 +103,8 @@ struct bcrt_server {
     std::unique_ptr<bcrt::CopyPool> pack, unpack;
     std::mutex m;
     std::condition_variable admit_cv, retire_cv, done_cv;
-    std::deque<Request*> pending;
-    std::deque<std::unique_ptr<Wave>> inflight;
+    std::unique_ptr<bcrt::WaveScheduler> sched;  // pending queue and ring space
+    std::deque<std::unique_ptr<Wave>> inflight;      // same waves, oldest first
     std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
     std::map<std::string, std::unique_ptr<Plan>> plans;
     std::mutex plan_m;
@@ -148,41 +121,18 @@ struct bcrt_server {
     void enqueue(Wave& w);
 };
 
-// Takes pending requests in arrival order while all three rings have room and
-// fewer than kInflight waves are on the device; groups them by plan.  Called
-// with the lock held.
+// Next wave from the scheduler (called with the lock held), with each plan's
+// requests as one group in the device arena.
 std::unique_ptr<Wave> bcrt_server::form_wave() {
-    if (pending.empty() || inflight.size() >= kInflight) return nullptr;
-    std::vector<Plan*> order;
-    std::map<Plan*, std::vector<Request*>> by_plan;
-    size_t need[3] = {0, 0, 0};
-    uint32_t taken = 0;
-    for (Request* r : pending) {
-        if (taken == max_wave) break;
-        if (!by_plan.count(r->plan) && order.size() == bcrt::kMaxGroups) break;
-        bool fits = true;
-        for (int k = 0; k < 3 && fits; ++k) {
-            size_t probe = 0;
-            fits = ring_alloc(inflight, k, cap[k], need[k] + footprint(r->plan, k), &probe);
-        }
-        if (!fits) break;
-        for (int k = 0; k < 3; ++k) need[k] += footprint(r->plan, k);
-        if (!by_plan.count(r->plan)) order.push_back(r->plan);
-        by_plan[r->plan].push_back(r);
-        ++taken;
-    }
-    if (taken == 0) return nullptr;
+    bcrt::SchedWave sw;
+    if (!sched->form(&sw)) return nullptr;
     auto w = std::make_unique<Wave>();
-    for (int k = 0; k < 3; ++k) {
-        ring_alloc(inflight, k, cap[k], need[k], &w->start[k]);
-        w->end[k] = w->start[k] + need[k];
-    }
-    pending.erase(pending.begin(), pending.begin() + taken);
+    for (int k = 0; k < bcrt::kRings; ++k) w->start[k] = sw.start[k], w->end[k] = sw.end[k];
     size_t off = w->start[kArena];
     uint32_t begin[5] = {0, 0, 0, 0, 0};
-    for (Plan* p : order) {
-        const auto& rs = by_plan[p];
-        const uint32_t n = uint32_t(rs.size());
+    for (size_t i = 0; i < sw.plans.size(); ++i) {
+        const Plan* p = static_cast<const Plan*>(sw.plans[i]);
+        const uint32_t n = uint32_t(sw.items[i].size());
         const size_t io_bytes = p->io_words * sizeof(uint32_t) * n;
         uint32_t* da = reinterpret_cast<uint32_t*>(arena + off);
         uint32_t* db = reinterpret_cast<uint32_t*>(arena + off + io_bytes);
@@ -197,7 +147,7 @@ std::unique_ptr<Wave> bcrt_server::form_wave() {
             begin[st] += bcrt::group_workgroups(d.t, n, st);
         }
         w->groups.push_back(d);
-        for (uint32_t i = 0; i < n; ++i) w->reqs.push_back(rs[i]);
+        for (const auto& it : sw.items[i]) w->reqs.push_back(static_cast<Request*>(it.user));
     }
     return w;
 }
@@ -244,7 +194,7 @@ void bcrt_server::admit_loop() {
     std::unique_lock<std::mutex> lk(m);
     for (;;) {
         std::unique_ptr<Wave> w;
-        admit_cv.wait(lk, [&] { return (w = form_wave()) != nullptr || (stop && pending.empty()); });
+        admit_cv.wait(lk, [&] { return (w = form_wave()) != nullptr || (stop && sched->pending() == 0); });
         if (!w) break;
         Wave* raw = w.get();
         // Published before the device work is enqueued, so the next wave's ring
@@ -304,6 +254,7 @@ void bcrt_server::retire_loop() {
         }
         if (!err.empty() && fatal.empty()) fatal = err;
         inflight.pop_front();
+        sched->retire_oldest();
         done_cv.notify_all();
         admit_cv.notify_one();
     }
@@ -333,6 +284,7 @@ int bcrt_server_create(bcrt_server** out, size_t hbm_budget_bytes, uint32_t max_
         for (auto& c : s->compute) hcheck(hipStreamCreateWithFlags(&c, hipStreamNonBlocking), "create stream");
         s->pack = std::make_unique<bcrt::CopyPool>(bcrt::pool_workers());
         s->unpack = std::make_unique<bcrt::CopyPool>(bcrt::pool_workers());
+        s->sched = std::make_unique<bcrt::WaveScheduler>(s->cap, max_wave, bcrt::kMaxGroups, kInflight);
     } catch (const std::exception& x) {
         if (err && errlen) std::snprintf(err, errlen, "%s", x.what());
         if (s->arena) (void)hipFree(s->arena);
@@ -375,13 +327,15 @@ int bcrt_server_submit(bcrt_server* s, uint32_t N, uint32_t W, uint32_t L, const
         }
         plan = it->second.get();
     }
+    bcrt::SchedItem item;
+    item.plan = plan;
+    for (int k = 0; k < bcrt::kRings; ++k) item.need[k] = footprint(plan, k);
     std::lock_guard<std::mutex> g(s->m);
-    for (int k = 0; k < 3; ++k)
-        if (footprint(plan, k) > s->cap[k]) {
-            s->last_error = "ByteCRT server: one product needs " + std::to_string(footprint(plan, kArena)) +
-                            " bytes of HBM, above the budget";
-            return BCRT_EINVAL;
-        }
+    if (!s->sched->fits_alone(item)) {
+        s->last_error = "ByteCRT server: one product needs " + std::to_string(item.need[kArena]) +
+                        " bytes of HBM, above the budget";
+        return BCRT_EINVAL;
+    }
     if (!s->fatal.empty()) {
         s->last_error = s->fatal;
         return BCRT_ERUNTIME;
@@ -393,7 +347,8 @@ int bcrt_server_submit(bcrt_server* s, uint32_t N, uint32_t W, uint32_t L, const
     r->b = b;
     r->c = c;
     r->submitted = Clock::now();
-    s->pending.push_back(r.get());
+    item.user = r.get();
+    s->sched->push(item);
     *ticket = r->id;
     s->requests.emplace(r->id, std::move(r));
     s->admit_cv.notify_one();
