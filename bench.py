@@ -103,6 +103,19 @@ def device_operands(torch, batch: int, n: int, q: int, seed: int):
     return out.reshape(batch, n, L).contiguous()
 
 
+def measured_traffic(stage: str):
+    """HBM bytes per B=128 dispatch of a stage kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated), or None."""
+    path = ROOT / "profiles" / "traffic_b128_n65536_w1747.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == "k_" + stage:
+            return int(f[1]) + int(f[2])
+    return None
+
+
 def stage_bytes(P: int, L: int):
     """Algorithmic HBM bytes per product for each kernel (compulsory reads+writes)."""
     R = N * P * 4          # one residue plane set for one operand
@@ -466,7 +479,7 @@ def main():
                 "unit": "TOP/s", "frac": stages[dom]["mfma_frac"], "traffic": None, "kernel": dom}
     else:
         roof = {"bound": "hbm", "achieved": stages[dom]["hbm_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": stages[dom]["hbm_frac"], "traffic": None, "kernel": dom,
+                "frac": stages[dom]["hbm_frac"], "traffic": measured_traffic(dom), "kernel": dom,
                 "algorithmic_bytes_per_launch": sb[dom] * B}
     ntt_ms = dur["fwd_cols"] + dur["middle"] + dur["inv_cols"]
     ntt_bytes = (sb["fwd_cols"] + sb["middle"] + sb["inv_cols"]) * B

@@ -20,7 +20,7 @@ LIB = PKG / "lib"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_io.cpp", "bytecrt_engine.cpp",
+SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_io.cpp", "bytecrt_trace.cpp", "bytecrt_engine.cpp",
            "bytecrt_server.cpp", "fherma_solve.cpp"]
 # harness_main.cpp is linked separately into lib/bytecrt_solution.
 
@@ -54,7 +54,8 @@ def build(verbose: bool = False) -> Path:
         objs.append(str(o))
     out = LIB / "libbytecrt.so"
     tmp = LIB / "libbytecrt.so.tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs,
+          "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, out)
     for o in objs:
         os.remove(o)

@@ -3,6 +3,7 @@
 #include "bytecrt_io.hpp"
 
 #include "bytecrt_copy.hpp"
+#include "bytecrt_trace.hpp"
 
 #include <sys/mman.h>
 
@@ -88,8 +89,12 @@ struct HostIo::Impl {
             try {
                 const uint32_t* src = pin_out + size_t(j.slot) * chunk * words;
                 for (uint32_t p = 0; p < j.pieces; ++p) {
-                    check(hipEventSynchronize(ev_piece[j.slot][p]), "wait output copy");
+                    {
+                        TraceScope tr("bcrt:wait");
+                        check(hipEventSynchronize(ev_piece[j.slot][p]), "wait output copy");
+                    }
                     if (skip) continue;
+                    TraceScope tr("bcrt:unpack");
                     const size_t off = p * j.piece, len = std::min(j.piece, j.bytes - off);
                     const uint32_t* s = src + off / sizeof(uint32_t);
                     if (j.dest)
@@ -169,6 +174,7 @@ uint32_t HostIo::chunk() const { return impl_->chunk; }
 void HostIo::run(const uint32_t* a, const uint32_t* b, uint32_t batch, uint32_t* c, bcrt_sink sink, void* ctx,
                  const Enqueue& enqueue) {
     Impl& s = *impl_;
+    TraceScope whole("bcrt:whole_call");
     const size_t words = s.words, chunk = s.chunk, slot_words = chunk * words;
     const uint32_t nchunks = uint32_t((batch + chunk - 1) / chunk);
     if (c) advise_huge(c, size_t(batch) * words * sizeof(uint32_t));
@@ -189,7 +195,10 @@ void HostIo::run(const uint32_t* a, const uint32_t* b, uint32_t batch, uint32_t*
             const uint32_t n = uint32_t(std::min<size_t>(chunk, batch - size_t(k) * chunk));
             // Slot k % kSlots is free once chunk k - kSlots has been unpacked:
             // its output copies, hence its compute and input copies, are done.
-            if (k >= kSlots && !wait_drained(base + k - kSlots + 1)) break;
+            if (k >= kSlots) {
+                TraceScope tr("bcrt:slot_wait");
+                if (!wait_drained(base + k - kSlots + 1)) break;
+            }
             const size_t bytes = size_t(n) * words * sizeof(uint32_t);
             uint32_t* hin = s.pin_in + size_t(slot) * 2 * slot_words;
             uint32_t* din = s.dev + size_t(slot) * 3 * slot_words;
@@ -199,7 +208,10 @@ void HostIo::run(const uint32_t* a, const uint32_t* b, uint32_t batch, uint32_t*
                 uint32_t* d = din + op * slot_words;
                 for (size_t off = 0; off < bytes; off += kInPiece) {
                     const size_t len = std::min(kInPiece, bytes - off), w = off / sizeof(uint32_t);
-                    s.pack->copy(h + w, src + w, len);
+                    {
+                        TraceScope tr("bcrt:pack");
+                        s.pack->copy(h + w, src + w, len);
+                    }
                     check(hipMemcpyAsync(d + w, h + w, len, hipMemcpyHostToDevice, s.h2d), "copy operand");
                 }
             }

@@ -30,6 +30,7 @@
 #include "bytecrt_copy.hpp"
 #include "bytecrt_kernels.hpp"
 #include "bytecrt_sched.hpp"
+#include "bytecrt_trace.hpp"
 
 namespace bcrt {
 // Tables and geometry of an engine (bytecrt_engine.cpp).
@@ -76,6 +77,7 @@ struct Wave {
     hipEvent_t inputs_ready = nullptr;
     hipEvent_t done = nullptr;
     bool enqueued = false;  // set under the lock once enqueue() returned
+    uint64_t trace = 0;     // range from admission to retirement
     std::string error;      // enqueue() failed
 };
 
@@ -197,6 +199,7 @@ void bcrt_server::admit_loop() {
         admit_cv.wait(lk, [&] { return (w = form_wave()) != nullptr || (stop && sched->pending() == 0); });
         if (!w) break;
         Wave* raw = w.get();
+        raw->trace = bcrt::trace_start("bcrt:wave");
         // Published before the device work is enqueued, so the next wave's ring
         // space is computed after this one's; the retire thread waits for
         // `enqueued` before touching it.
@@ -206,6 +209,7 @@ void bcrt_server::admit_loop() {
         lk.unlock();
         std::string err;
         try {
+            bcrt::TraceScope tr("bcrt:enqueue_wave");
             enqueue(*raw);
         } catch (const std::exception& x) {
             err = x.what();
@@ -229,10 +233,12 @@ void bcrt_server::retire_loop() {
         std::string err = w.error;
         lk.unlock();
         if (err.empty()) {
+            bcrt::TraceScope tr("bcrt:wait");
             const hipError_t q = hipEventSynchronize(w.done);
             if (q != hipSuccess) err = std::string("ByteCRT server wave failed: ") + hipGetErrorString(q);
         }
         if (err.empty()) {
+            bcrt::TraceScope tr("bcrt:unpack");
             size_t out_off = w.start[kPinOut], ri = 0;
             for (const auto& g : w.groups) {
                 const size_t bytes = size_t(g.t.N) * g.t.L * sizeof(uint32_t);
@@ -245,6 +251,7 @@ void bcrt_server::retire_loop() {
         const auto now = Clock::now();
         if (w.done) (void)hipEventDestroy(w.done);
         if (w.inputs_ready) (void)hipEventDestroy(w.inputs_ready);
+        bcrt::trace_stop(w.trace);
         lk.lock();
         for (Request* r : w.reqs) {
             r->latency = std::chrono::duration<double>(now - r->submitted).count();

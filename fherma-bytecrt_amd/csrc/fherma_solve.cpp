@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/bytecrt.h"
+#include "bytecrt_trace.hpp"
 
 #ifndef MADV_POPULATE_WRITE
 #define MADV_POPULATE_WRITE 23
@@ -109,6 +110,7 @@ void* fherma_init(const fherma::Point& p) {
 
 fherma::Outputs fherma_run(void* state, const fherma::Inputs& in) {
     auto& s = *static_cast<Solver*>(state);
+    bcrt::TraceScope trace("bcrt:fherma_run");
     const size_t total = size_t(s.B) * s.N * s.L;
     if (in.a.data.size() != total || in.b.data.size() != total)
         throw std::runtime_error("ByteCRT input coefficient count");
