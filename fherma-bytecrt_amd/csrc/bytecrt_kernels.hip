@@ -147,60 +147,80 @@ size_t enc_smem_bytes(const DevTables& t) {
 }
 
 // One encode workgroup: tile bx of operand op of product prod.
-template <int KS>
-__device__ __forceinline__ void encode_wg(const DevTables& t, const uint32_t* __restrict__ a,
-                                          const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
-                                          uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem) {
-    uint32_t* enc_tile = reinterpret_cast<uint32_t*>(smem);
-    // Per-prime reduction constants {off, 1/p rounded down, p, off mod 2^32}.
+// Encode tile bx: TR consecutive r (index mod 256) by TS consecutive s (index
+// div 256), TS = min(32, N1): each s brings TR consecutive coefficients
+// (TR * L contiguous input words), each (prime, r) gets TS consecutive plane
+// positions.  Tile row m is (r0 + m / TS, s0 + m % TS).
+struct EncGeom {
+    uint32_t ts_log, TS, TR, r0, s0, row_words, stride_words;
+    __device__ __forceinline__ EncGeom(const DevTables& t, uint32_t bx) {
+        ts_log = min(5u, t.n1);
+        TS = 1u << ts_log;
+        TR = kEncCoefs >> ts_log;
+        const uint32_t rtiles = t.N2 / TR;
+        r0 = (bx % rtiles) * TR;
+        s0 = (bx / rtiles) * TS;
+        row_words = t.Kenc / 4;
+        stride_words = row_words + 4;
+    }
+};
+
+constexpr uint32_t kEncSteps = kEncCoefs * 64 / kThreads;  // input words per thread (L <= 64)
+
+// Issues every input load of the tile into registers.  Word v of the tile:
+// group gi = v / G (one s), offset o = v % G within the group's TR * L
+// contiguous words; division by the magic ceil(2^32 / d) is exact for
+// v * (magic * d - 2^32) < 2^32 (v < 2^13).  Loads are unconditional (past the
+// end a thread re-reads the tile's last word) so all are in flight at once.
+__device__ __forceinline__ void enc_load(const DevTables& t, const uint32_t* __restrict__ in, const EncGeom& e,
+                                         uint32_t (&regs)[kEncSteps]) {
+    const uint32_t G = e.TR * t.L, total = kEncCoefs * t.L;
+    const uint32_t mg = 0xFFFFFFFFu / G + 1;
+    const uint32_t* src = in + (size_t(e.r0) + size_t(t.N2) * e.s0) * t.L;
+    const uint32_t group_stride = t.N2 * t.L;  // 32-bit offsets: N * L < 2^22
+#pragma unroll
+    for (uint32_t it = 0; it < kEncSteps; ++it) {
+        const uint32_t v = min(threadIdx.x + kThreads * it, total - 1);
+        const uint32_t gi = __umulhi(v, mg);
+        regs[it] = src[gi * group_stride + (v - gi * G)];
+    }
+}
+
+// Writes the loaded words into the LDS tile as biased bytes (x ^ 0x80 =
+// x - 128, for the signed MFMA); padding limbs L .. Kenc/4 - 1 hold biased zeros.
+__device__ __forceinline__ void enc_store(const DevTables& t, const EncGeom& e, const uint32_t (&regs)[kEncSteps],
+                                          uint32_t* enc_tile) {
+    const uint32_t G = e.TR * t.L, total = kEncCoefs * t.L;
+    const uint32_t mg = 0xFFFFFFFFu / G + 1, ml = 0xFFFFFFFFu / t.L + 1;
+#pragma unroll
+    for (uint32_t it = 0; it < kEncSteps; ++it) {
+        const uint32_t v = threadIdx.x + kThreads * it;
+        if (v < total) {
+            const uint32_t gi = __umulhi(v, mg), o = v - gi * G;
+            const uint32_t ro = __umulhi(o, ml), limb = o - ro * t.L;
+            enc_tile[((ro << e.ts_log) + gi) * e.stride_words + limb] = regs[it] ^ 0x80808080u;
+        }
+    }
+    const uint32_t pad = e.row_words - t.L;
+    for (uint32_t i = threadIdx.x; i < kEncCoefs * pad; i += kThreads)
+        enc_tile[(i / pad) * e.stride_words + t.L + i % pad] = 0x80808080u;
+}
+
+// The per-prime reduction constants {off, 1/p rounded down, p, off mod 2^32}
+// into LDS (after the data tile).
+__device__ __forceinline__ EncPrime* enc_constants(const DevTables& t, uint8_t* smem) {
     EncPrime* cst = reinterpret_cast<EncPrime*>(smem + size_t(kEncCoefs) * (t.Kenc + 16));
     for (uint32_t j = threadIdx.x; j < t.P; j += kThreads)
         cst[j] = reinterpret_cast<const EncPrime*>(t.enc_prime)[j];
-    const uint32_t* in = (op ? b : a) + size_t(prod) * t.N * t.L;
-    // The tile is TR consecutive r (index mod 256) by TS consecutive s (index
-    // div 256), TS = min(32, N1): each s brings TR consecutive coefficients
-    // (TR * L contiguous input words), each (prime, r) gets TS consecutive
-    // plane positions.  Tile row m is (r0 + m / TS, s0 + m % TS).
-    const uint32_t ts_log = min(5u, t.n1), TS = 1u << ts_log, TR = kEncCoefs >> ts_log;
-    const uint32_t rtiles = t.N2 / TR;
-    const uint32_t r0 = (bx % rtiles) * TR, s0 = (bx / rtiles) * TS;
-    const uint32_t row_words = t.Kenc / 4;
-    const uint32_t stride_words = row_words + 4;
-    {
-        // Word v of the tile: group gi = v / G (one s), offset o = v % G within
-        // the group's TR * L contiguous words.  Division by the magic
-        // ceil(2^32 / d) is exact for v * (magic * d - 2^32) < 2^32 (v < 2^13).
-        const uint32_t G = TR * t.L, total = kEncCoefs * t.L;
-        const uint32_t mg = 0xFFFFFFFFu / G + 1, ml = 0xFFFFFFFFu / t.L + 1;
-        const uint32_t* src = in + (size_t(r0) + size_t(t.N2) * s0) * t.L;
-        const uint32_t group_stride = t.N2 * t.L;  // 32-bit offsets: N * L < 2^22
-        constexpr uint32_t kSteps = kEncCoefs * 64 / kThreads;  // L <= 64
-        // Loads are unconditional (past the end a thread re-reads the tile's
-        // last word) so that all of them are in flight before the first use.
-        uint32_t regs[kSteps];
-#pragma unroll
-        for (uint32_t it = 0; it < kSteps; ++it) {
-            const uint32_t v = min(threadIdx.x + kThreads * it, total - 1);
-            const uint32_t gi = __umulhi(v, mg);
-            regs[it] = src[gi * group_stride + (v - gi * G)];
-        }
-        // Bytes are stored biased (x ^ 0x80 = x - 128) for the signed MFMA.
-#pragma unroll
-        for (uint32_t it = 0; it < kSteps; ++it) {
-            const uint32_t v = threadIdx.x + kThreads * it;
-            if (v < total) {
-                const uint32_t gi = __umulhi(v, mg), o = v - gi * G;
-                const uint32_t ro = __umulhi(o, ml), limb = o - ro * t.L;
-                enc_tile[((ro << ts_log) + gi) * stride_words + limb] = regs[it] ^ 0x80808080u;
-            }
-        }
-        // Padding limbs L .. Kenc/4 - 1 hold biased zero bytes.
-        const uint32_t pad = row_words - t.L;
-        for (uint32_t i = threadIdx.x; i < kEncCoefs * pad; i += kThreads)
-            enc_tile[(i / pad) * stride_words + t.L + i % pad] = 0x80808080u;
-    }
-    __syncthreads();
+    return cst;
+}
 
+// GEMM and reduction of the tile in LDS, residues to the planes.
+template <int KS>
+__device__ __forceinline__ void enc_compute(const DevTables& t, uint32_t* __restrict__ planes, const EncGeom& e,
+                                            uint32_t op, uint32_t prod, const uint8_t* smem, const EncPrime* cst) {
+    const uint32_t* enc_tile = reinterpret_cast<const uint32_t*>(smem);
+    const uint32_t ts_log = e.ts_log, TS = e.TS, r0 = e.r0, s0 = e.s0, stride_words = e.stride_words;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t row = lane & 31, kh = lane >> 5;
     const uint8_t* tile_bytes = reinterpret_cast<const uint8_t*>(enc_tile);
@@ -215,44 +235,64 @@ __device__ __forceinline__ void encode_wg(const DevTables& t, const uint32_t* __
         v4i A[KS];
 #pragma unroll
         for (int s2 = 0; s2 < KS; ++s2) A[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
-        {
-            v16i acc[kEncChains];
+        // constants of this lane's four primes j = 8 pt + 2 g + kh (clamped;
+        // the stores of primes past P are masked)
+        double off[4], inv[4], pd[4];
 #pragma unroll
-            for (int c = 0; c < kEncChains; ++c) acc[c] = v16i{};
+        for (int g = 0; g < 4; ++g) {
+            const EncPrime c = cst[min(pt * 8 + 2 * g + kh, t.P - 1)];
+            off[g] = c.off;
+            inv[g] = c.inv;
+            pd[g] = double(c.p);
+        }
+        v16i acc[kEncChains];
 #pragma unroll
-            for (int s2 = 0; s2 < KS; ++s2) {
-                v4i B[kEncChains];
+        for (int c = 0; c < kEncChains; ++c) acc[c] = v16i{};
 #pragma unroll
-                for (int c = 0; c < kEncChains; ++c)
-                    B[c] = *reinterpret_cast<const v4i*>(tile_bytes + ((c0 + c) * 32 + row) * stride_bytes +
-                                                         16 * kh + 32 * s2);
+        for (int s2 = 0; s2 < KS; ++s2) {
+            v4i B[kEncChains];
 #pragma unroll
-                for (int c = 0; c < kEncChains; ++c)
-                    acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[s2], B[c], acc[c], 0, 0, 0);
-            }
+            for (int c = 0; c < kEncChains; ++c)
+                B[c] = *reinterpret_cast<const v4i*>(tile_bytes + ((c0 + c) * 32 + row) * stride_bytes + 16 * kh +
+                                                     32 * s2);
 #pragma unroll
-            for (int c = 0; c < kEncChains; ++c) {
+            for (int c = 0; c < kEncChains; ++c)
+                acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[s2], B[c], acc[c], 0, 0, 0);
+        }
 #pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const uint32_t j = pt * 8 + 2 * g + kh;
-                    if (j < t.P) {
-                        const EncPrime cp = cst[j];
-                        // digit sums |C| < 2^22: the two byte pairs fit int32
-                        const int32_t lo = acc[c][4 * g] + acc[c][4 * g + 1] * 256;
-                        const int32_t hi = acc[c][4 * g + 2] + acc[c][4 * g + 3] * 256;
-                        // u = lo + 2^16 hi + off, 0 <= u < 2^47: exact in FP64; cp.inv <= 1/p
-                        // makes qd = floor(u/p) or one less, so r = u - qd p is in [0, 2p).
-                        const double ud = fma(double(hi), 65536.0, double(lo)) + cp.off;
-                        const uint32_t qd = uint32_t(ud * cp.inv);
-                        const uint32_t u32 = uint32_t(lo) + (uint32_t(hi) << 16) + cp.off_lo;
-                        const uint32_t r = u32 - qd * cp.p;
-                        const uint32_t m = (c0 + c) * 32 + row;
-                        out[j * t.N + (r0 + (m >> ts_log)) * t.N1 + s0 + (m & (TS - 1))] = min(r, r - cp.p);
-                    }
-                }
+        for (int c = 0; c < kEncChains; ++c) {
+            const uint32_t m = (c0 + c) * 32 + row;
+            const uint32_t f = (r0 + (m >> ts_log)) * t.N1 + s0 + (m & (TS - 1));
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                // digit sums |C| < 2^22: the two byte pairs fit int32
+                const int32_t lo = acc[c][4 * g] + acc[c][4 * g + 1] * 256;
+                const int32_t hi = acc[c][4 * g + 2] + acc[c][4 * g + 3] * 256;
+                // u = lo + 2^16 hi + off, 0 <= u < 2^47, exact in FP64; inv <= 1/p makes
+                // qd = floor(u / p) or one less, so u - qd p (exact: both below 2^47)
+                // is in [0, 2p).
+                const double ud = fma(double(hi), 65536.0, double(lo)) + off[g];
+                const double qd = __builtin_trunc(ud * inv[g]);
+                const uint32_t r = uint32_t(fma(-qd, pd[g], ud));
+                const uint32_t j = pt * 8 + 2 * g + kh;
+                if (j < t.P) out[j * t.N + f] = min(r, r - uint32_t(pd[g]));
             }
         }
     }
+}
+
+// One encode tile (bx, op, prod).
+template <int KS>
+__device__ __forceinline__ void encode_wg(const DevTables& t, const uint32_t* __restrict__ a,
+                                          const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
+                                          uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem) {
+    const EncPrime* cst = enc_constants(t, smem);
+    const EncGeom e(t, bx);
+    uint32_t regs[kEncSteps];
+    enc_load(t, (op ? b : a) + size_t(prod) * t.N * t.L, e, regs);
+    enc_store(t, e, regs, reinterpret_cast<uint32_t*>(smem));
+    __syncthreads();
+    enc_compute<KS>(t, planes, e, op, prod, smem, cst);
 }
 
 // Encode with the k-step count (Kenc / 32, 1..8) as a compile-time constant.
@@ -477,13 +517,14 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
 // which is floor(z/q) or one less, so one conditional subtraction finishes.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kCrtThreads = 256;
-constexpr uint32_t kCrtMaxK = (4 * (kMaxPrimesCrt + 1) + 31) / 32;  // k steps of 32
 
 size_t crt_smem_bytes(const DevTables& t) {
     // term rows (later reused for the digit rows) + int64 limb sums
     return size_t(kCrtCoefs) * (t.Kcrt + 16) + size_t(t.L + 1) * kCrtCoefs * 8;
 }
 
+// CRT for 64 coefficients with the k-loop of the GEMM (Kcrt / 32 steps) fixed.
+template <int KS>
 __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __restrict__ planes,
                                        uint32_t* __restrict__ out, uint32_t bx, uint32_t prod,
                                        uint8_t* crt_smem) {
@@ -501,7 +542,7 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
     {
         // lane c gathers four consecutive terms of its coefficient (coalesced per
         // plane); every load of a thread is issued before its LDS stores.
-        constexpr uint32_t kIters = (kCrtMaxK * 2 + 3) / 4;
+        constexpr uint32_t kIters = (KS * 2 + 3) / 4;
         const uint32_t c = threadIdx.x & 63, g0 = threadIdx.x >> 6;
         uint32_t v[kIters][4];
 #pragma unroll
@@ -509,15 +550,19 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
             const uint32_t g = g0 + 4 * it;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const uint32_t j = 4 * g + k;
-                v[it][k] = (g < groups && j < P) ? pa[size_t(j) * t.N + c] : 0u;
+                const uint32_t j = min(4 * g + k, P - 1);  // clamped: every load unconditional
+                v[it][k] = pa[size_t(j) * t.N + c];
             }
         }
 #pragma unroll
         for (uint32_t it = 0; it < kIters; ++it) {
             const uint32_t g = g0 + 4 * it;
-            if (g < groups)
-                *reinterpret_cast<uint4*>(tmw + c * tw + 4 * g) = make_uint4(v[it][0], v[it][1], v[it][2], v[it][3]);
+            if (g < groups) {
+                uint32_t w4[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) w4[k] = 4 * g + k < P ? v[it][k] : 0u;
+                *reinterpret_cast<uint4*>(tmw + c * tw + 4 * g) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+            }
         }
     }
     __syncthreads();
@@ -534,22 +579,18 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t row = lane & 31, kh = lane >> 5;
     const v4i bias = {int(0x80808080u), int(0x80808080u), int(0x80808080u), int(0x80808080u)};
-    const uint32_t ksteps = t.Kcrt / 32;
     for (uint32_t mt = wave; mt < t.Mcrt / 32; mt += 4) {
         const int8_t* wrow = t.wcrt + size_t(mt * 32 + row) * t.Kcrt + 16 * kh;
-        v4i A[kCrtMaxK];
+        v4i A[KS];
 #pragma unroll
-        for (uint32_t k = 0; k < kCrtMaxK; ++k)
-            A[k] = k < ksteps ? *reinterpret_cast<const v4i*>(wrow + 32 * k) : v4i{};
+        for (int k = 0; k < KS; ++k) A[k] = *reinterpret_cast<const v4i*>(wrow + 32 * k);
         v16i acc0 = v16i{}, acc1 = v16i{};
 #pragma unroll
-        for (uint32_t k = 0; k < kCrtMaxK; ++k) {
-            if (k < ksteps) {
-                const v4i B0 = *reinterpret_cast<const v4i*>(tmat + row * tstride + 32 * k + 16 * kh) ^ bias;
-                const v4i B1 = *reinterpret_cast<const v4i*>(tmat + (32 + row) * tstride + 32 * k + 16 * kh) ^ bias;
-                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B0, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B1, acc1, 0, 0, 0);
-            }
+        for (int k = 0; k < KS; ++k) {
+            const v4i B0 = *reinterpret_cast<const v4i*>(tmat + row * tstride + 32 * k + 16 * kh) ^ bias;
+            const v4i B1 = *reinterpret_cast<const v4i*>(tmat + (32 + row) * tstride + 32 * k + 16 * kh) ^ bias;
+            acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B1, acc1, 0, 0, 0);
         }
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -626,6 +667,31 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
             const uint32_t n = (f >> t.n1) + t.N2 * (f & (t.N1 - 1));
             dst[size_t(n) * L + w] = dig[cc * DS + w];
         }
+    }
+}
+
+// CRT with the k-step count (Kcrt / 32, 1..17) as a compile-time constant.
+__device__ __forceinline__ void crt_dispatch(const DevTables& t, const uint32_t* __restrict__ planes,
+                                             uint32_t* __restrict__ out, uint32_t bx, uint32_t prod,
+                                             uint8_t* smem) {
+    switch (t.Kcrt / 32) {
+        case 1: crt_wg<1>(t, planes, out, bx, prod, smem); break;
+        case 2: crt_wg<2>(t, planes, out, bx, prod, smem); break;
+        case 3: crt_wg<3>(t, planes, out, bx, prod, smem); break;
+        case 4: crt_wg<4>(t, planes, out, bx, prod, smem); break;
+        case 5: crt_wg<5>(t, planes, out, bx, prod, smem); break;
+        case 6: crt_wg<6>(t, planes, out, bx, prod, smem); break;
+        case 7: crt_wg<7>(t, planes, out, bx, prod, smem); break;
+        case 8: crt_wg<8>(t, planes, out, bx, prod, smem); break;
+        case 9: crt_wg<9>(t, planes, out, bx, prod, smem); break;
+        case 10: crt_wg<10>(t, planes, out, bx, prod, smem); break;
+        case 11: crt_wg<11>(t, planes, out, bx, prod, smem); break;
+        case 12: crt_wg<12>(t, planes, out, bx, prod, smem); break;
+        case 13: crt_wg<13>(t, planes, out, bx, prod, smem); break;
+        case 14: crt_wg<14>(t, planes, out, bx, prod, smem); break;
+        case 15: crt_wg<15>(t, planes, out, bx, prod, smem); break;
+        case 16: crt_wg<16>(t, planes, out, bx, prod, smem); break;
+        default: crt_wg<17>(t, planes, out, bx, prod, smem); break;
     }
 }
 
@@ -724,7 +790,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt(DevTables t, const uint32_t
     extern __shared__ __align__(16) uint8_t smem_crt[];
     uint32_t x, y, z;
     single_xyz(t, 4, &x, &y, &z);
-    crt_wg(t, planes, out, x, z, smem_crt);
+    crt_dispatch(t, planes, out, x, z, smem_crt);
 }
 
 // Finds this workgroup's group and its (x, y, z) in that group's grid, or
@@ -819,7 +885,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_groups(const GroupTable g) 
     uint32_t k;
     if (!locate(g, 4, &k, &x, &y, &z)) return;
     const GroupDesc& d = g.g[k];
-    crt_wg(d.t, d.args.planes, d.args.out, x, z, smem_crt_g);
+    crt_dispatch(d.t, d.args.planes, d.args.out, x, z, smem_crt_g);
 }
 
 bool shapes_ok(const DevTables& t, const BatchArgs& args) {
