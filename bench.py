@@ -286,13 +286,18 @@ def serving_leg(pkg, qps: float, seconds: float, budget_gib: float):
     ok = [False] * count
     submitted = threading.Semaphore(0)
 
+    # result buffers are recycled per geometry, as a serving client would
+    free = [[np.zeros((n, (w + 31) // 32), dtype=np.uint32) for _ in range(32)] for n, w in SERVE_GEOMS]
+    free_lock = threading.Lock()
+
     def waiter():
         for i in range(count):
             submitted.acquire()
             c, lat[i] = srv.wait(tickets[i])
             done_at[i] = time.perf_counter()
-            n, w, q, entries = pools[kinds[i]]
-            ok[i] = bool(np.array_equal(c, entries[picks[i]][2]))
+            ok[i] = bool(np.array_equal(c, pools[kinds[i]][3][picks[i]][2]))
+            with free_lock:
+                free[kinds[i]].append(c)
 
     th = threading.Thread(target=waiter)
     th.start()
@@ -305,7 +310,9 @@ def serving_leg(pkg, qps: float, seconds: float, budget_gib: float):
             time.sleep(due - now)
         n, w, q, entries = pools[kinds[i]]
         a, b, _ = entries[picks[i]]
-        tickets[i] = srv.submit(n, w, q, a, b)
+        with free_lock:
+            c = free[kinds[i]].pop() if free[kinds[i]] else np.empty((n, (w + 31) // 32), dtype=np.uint32)
+        tickets[i] = srv.submit(n, w, q, a, b, c)
         submitted.release()
     th.join()
     span = max(done_at) - t0

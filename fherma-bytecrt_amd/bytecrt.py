@@ -280,6 +280,8 @@ class Server:
             raise RuntimeError("ByteCRT input coefficient count")
         if c is None:
             c = np.empty((N, L), dtype=np.uint32)
+        elif c.dtype != np.uint32 or not c.flags.c_contiguous or c.size != N * L:
+            raise ValueError("ByteCRT server output must be a contiguous uint32 array of N*L words")
         qw = limbs_of(q, L)
         t = ctypes.c_uint64()
         self._check(load_library().bcrt_server_submit(
