@@ -82,7 +82,8 @@ class CopyPool {
 // the page size changes; failure (no THP) is harmless.
 void advise_huge(void* p, size_t bytes);
 
-// Workers for one copy pool: half the hardware threads, less the caller, at most 7.
+// Workers for one copy pool: half the hardware threads, less the caller, at most
+// 7; the environment variable BCRT_COPY_THREADS overrides the pool size.
 unsigned pool_workers();
 
 }  // namespace bcrt

@@ -8,6 +8,7 @@
 #include <sys/mman.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -41,6 +42,11 @@ void advise_huge(void* p, size_t bytes) {
 }
 
 unsigned pool_workers() {
+    // BCRT_COPY_THREADS (1..64) sets the threads of each pool, the caller included.
+    if (const char* v = std::getenv("BCRT_COPY_THREADS")) {
+        const long n = std::strtol(v, nullptr, 10);
+        if (n >= 1 && n <= 64) return unsigned(n - 1);
+    }
     const unsigned hw = std::max(2u, std::thread::hardware_concurrency());
     return std::min(7u, hw / 2 - 1);
 }

@@ -102,6 +102,30 @@ __device__ __forceinline__ void ct_round(double (&v)[16], const double* __restri
     }
 }
 
+// ct_round on two operands of the same prime and block at once: each twiddle is
+// loaded once and the two butterfly chains are independent.
+template <int K>
+__device__ __forceinline__ void ct_round2(double (&v)[16], double (&u)[16], const double* __restrict__ z,
+                                          uint32_t mult, uint32_t e0, uint32_t o0, double p, double pinv) {
+#pragma unroll
+    for (int l = 0; l < K; ++l) {
+        const int half = 1 << (K - 1 - l);
+#pragma unroll
+        for (int sb = 0; sb < (16 >> K); ++sb) {
+#pragma unroll
+            for (int g = 0; g < (1 << l); ++g) {
+                const double w = z[(mult << (e0 + l)) + ((o0 + sb) << l) + g];
+#pragma unroll
+                for (int x = 0; x < half; ++x) {
+                    const int i0 = (sb << K) + g * 2 * half + x;
+                    ct_bfly(v[i0], v[i0 + half], w, p, pinv);
+                    ct_bfly(u[i0], u[i0 + half], w, p, pinv);
+                }
+            }
+        }
+    }
+}
+
 template <int K>
 __device__ __forceinline__ void gs_round(double (&v)[16], const double* __restrict__ z, uint32_t mult,
                                          uint32_t e0, uint32_t o0, double p, double pinv) {
@@ -447,37 +471,52 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
     const uint32_t mult = t.N1 + b0 + bl;
     const double* zf = t.zf + size_t(j) * t.N;
     const double* zi = t.zi + size_t(j) * t.N;
+    // The exchange buffer holds both operands as canonical u32 (A in the first
+    // half, B in the second) for the forward sub-levels, then the product as FP64.
+    uint32_t* ua = reinterpret_cast<uint32_t*>(sx);
+    uint32_t* ub = ua + kMidBlocks * kMidS;
+    uint32_t* ra = ua + bl * kMidS;
+    uint32_t* rb = ub + bl * kMidS;
     double* rx = sx + bl * kMidS;
     double v[16], w[16];
-    // Both operands' tiles are fetched up front; B waits in registers while A
-    // goes through forward sub-levels 0..3 (positions tau + 16x), then 4..7 (16 tau + x).
-    uint4 qa[4], qb[4];
-    mid_fetch(qa, pa, t.N1, b0);
-    mid_fetch(qb, pb, t.N1, b0);
-    mid_store_lds(sx, qa);
+    {
+        uint4 qa[4], qb[4];
+        mid_fetch(qa, pa, t.N1, b0);
+        mid_fetch(qb, pb, t.N1, b0);
+        const uint32_t o = skew(threadIdx.x);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ua[(4 * k + 0) * kMidS + o] = qa[k].x;
+            ua[(4 * k + 1) * kMidS + o] = qa[k].y;
+            ua[(4 * k + 2) * kMidS + o] = qa[k].z;
+            ua[(4 * k + 3) * kMidS + o] = qa[k].w;
+            ub[(4 * k + 0) * kMidS + o] = qb[k].x;
+            ub[(4 * k + 1) * kMidS + o] = qb[k].y;
+            ub[(4 * k + 2) * kMidS + o] = qb[k].z;
+            ub[(4 * k + 3) * kMidS + o] = qb[k].w;
+        }
+    }
     __syncthreads();
+    // forward sub-levels 0..3 (positions tau + 16x) of A and B together
 #pragma unroll
-    for (int x = 0; x < 16; ++x) v[x] = rx[skew(tau + 16 * x)];
-    ct_round<4>(v, zf, mult, 0, 0, p, pinv);
+    for (int x = 0; x < 16; ++x) {
+        v[x] = double(ra[skew(tau + 16 * x)]);
+        w[x] = double(rb[skew(tau + 16 * x)]);
+    }
+    ct_round2<4>(v, w, zf, mult, 0, 0, p, pinv);
 #pragma unroll
-    for (int x = 0; x < 16; ++x) rx[skew(tau + 16 * x)] = v[x];
+    for (int x = 0; x < 16; ++x) {
+        ra[skew(tau + 16 * x)] = canon(v[x], p, pinv);
+        rb[skew(tau + 16 * x)] = canon(w[x], p, pinv);
+    }
     __syncthreads();
+    // sub-levels 4..7 (positions 16 tau + x)
 #pragma unroll
-    for (int x = 0; x < 16; ++x) v[x] = rx[skew(16 * tau + x)];
-    ct_round<4>(v, zf, mult, 4, tau, p, pinv);
-    __syncthreads();
-    // operand B
-    mid_store_lds(sx, qb);
-    __syncthreads();
-#pragma unroll
-    for (int x = 0; x < 16; ++x) w[x] = rx[skew(tau + 16 * x)];
-    ct_round<4>(w, zf, mult, 0, 0, p, pinv);
-#pragma unroll
-    for (int x = 0; x < 16; ++x) rx[skew(tau + 16 * x)] = w[x];
-    __syncthreads();
-#pragma unroll
-    for (int x = 0; x < 16; ++x) w[x] = rx[skew(16 * tau + x)];
-    ct_round<4>(w, zf, mult, 4, tau, p, pinv);
+    for (int x = 0; x < 16; ++x) {
+        v[x] = double(ra[skew(16 * tau + x)]);
+        w[x] = double(rb[skew(16 * tau + x)]);
+    }
+    ct_round2<4>(v, w, zf, mult, 4, tau, p, pinv);
     // pointwise product (|v|, |w| < 2^34) and inverse sub-levels 7..4
 #pragma unroll
     for (int x = 0; x < 16; ++x) v[x] = mul_mod(v[x], w[x], p, pinv);
