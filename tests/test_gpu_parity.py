@@ -181,6 +181,31 @@ def test_batch_graph_and_whole_call(pkg, orc, gu):
     np.testing.assert_array_equal(gu.from_dev(out).reshape(B, N, e.L), batched)
 
 
+@pytest.mark.skip(reason="bcrt_autotune withdrawn: the process died with SIGSEGV on the host (no GPU fault) "
+                         "when it ran at B=128 N=65536 W=1747; cause not found, so the entry point was taken out")
+def test_autotune_keeps_results(pkg, gu):
+    """The autotuner picks one of its pipelining settings; results after it
+    equal those before."""
+    N, W, B = 32768, 868, 12
+    q = pkg.datagen.modulus(W, 55)
+    e = pkg.Engine(N, W, q, max_batch=B)
+    a = np.stack([pkg.datagen.operands(N, q, 500 + i) for i in range(B)])
+    b = np.stack([pkg.datagen.operands(N, q, 600 + i) for i in range(B)])
+    da, db = gu.to_dev(a), gu.to_dev(b)
+    out = gu.empty_dev(B * N * e.L)
+    e.run_device(da, db, out, B)
+    before = gu.from_dev(out)
+    choice = e.autotune(B)
+    assert choice["ms"] > 0 and 1 <= choice["streams"] <= 8 and 1 <= choice["chunk"] <= B
+    for graph in (False, True):
+        out.zero_()
+        gu.sync()
+        e.run_device(da, db, out, B, graph=graph)
+        np.testing.assert_array_equal(gu.from_dev(out), before)
+    with pytest.raises(ValueError):
+        e.autotune(B + 1)
+
+
 def test_whole_call_multi_chunk(pkg, gu):
     """The host whole call at N=65,536 with B=5: three chunks through the
     staging ring, several pieces per copy, equal to the device-resident run."""
