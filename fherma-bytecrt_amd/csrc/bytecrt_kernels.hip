@@ -551,19 +551,35 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
 // exact value is within 1/4 of an integer; the FP64 error is below 2^-37).  Then
 //   z = sum_{j<=P, a<4} byte_a(t_j) * W[j,a],  W[j,a] = 256^a w_j mod q,
 // as an int8 MFMA GEMM over balanced signed weight digits and biased bytes
-// (wave w computes digit-row tiles w, w+4, ...), a signed carry chain per
+// (wave w computes digit-row tiles w and w+4), a signed carry chain per
 // coefficient, and r = z - Q q with Q = floor(z_hi / (q_hi + 1)) (exact),
 // which is floor(z/q) or one less, so one conditional subtraction finishes.
+//
+// One LDS region serves three phases: the term rows (GEMM operand), then the
+// int64 limb sums [L+1][64] (written once every wave has left the GEMM; the
+// accumulators wait in registers across the barrier), then the output digits
+// [L][65] u32 written in place over the limb sums by the carry chain.  Digit
+// (i, c) occupies bytes [260 i + 4 c, +4), inside [260 i, 260 i + 256), and the
+// limb sums not yet read at step i start at byte 512 (i + 1), so no digit write
+// reaches an unread sum.  The stride 65 keeps the output stage's reads (65
+// digits apart across a wave) on distinct banks.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kCrtThreads = 256;
+constexpr uint32_t kCrtDigitStride = kCrtCoefs + 1;
 
 size_t crt_smem_bytes(const DevTables& t) {
-    // term rows (later reused for the digit rows) + int64 limb sums
-    return size_t(kCrtCoefs) * (t.Kcrt + 16) + size_t(t.L + 1) * kCrtCoefs * 8;
+    return std::max(size_t(kCrtCoefs) * (t.Kcrt + 16), size_t(t.L + 1) * kCrtCoefs * 8);
 }
 
-// CRT for 64 coefficients with the k-loop of the GEMM (Kcrt / 32 steps) fixed.
-template <int KS>
+// Row tiles per wave the instantiations cover: the digit-row tiles Mcrt / 32
+// are at most 4 when Kcrt / 32 <= 8 and at most 8 otherwise.
+__host__ __device__ constexpr uint32_t crt_tiles_per_wave(uint32_t ks, uint32_t mt) {
+    return ks <= 8 ? 1 : (ks == 9 && mt <= 4 ? 1 : 2);
+}
+
+// CRT for 64 coefficients with the k-loop of the GEMM (Kcrt / 32 steps) and the
+// row tiles per wave (KT, four waves) fixed.
+template <int KS, int KT>
 __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __restrict__ planes,
                                        uint32_t* __restrict__ out, uint32_t bx, uint32_t prod,
                                        uint8_t* crt_smem) {
@@ -571,11 +587,10 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
     const uint32_t tstride = t.Kcrt + 16;  // bytes per term row (odd multiple of 16: conflict-free b128)
     const uint32_t tw = tstride / 4;
     const uint32_t groups = t.Kcrt / 16;   // 4-term groups per row
-    const uint32_t DS = L | 1;             // odd digit-row stride
     uint8_t* tmat = crt_smem;
     uint32_t* tmw = reinterpret_cast<uint32_t*>(crt_smem);
-    uint32_t* dig = tmw;  // [64][DS], reuses the term rows once the GEMM is done
-    int64_t* lo = reinterpret_cast<int64_t*>(crt_smem + kCrtCoefs * tstride);  // [L+1][64]
+    int64_t* lo = reinterpret_cast<int64_t*>(crt_smem);  // [L+1][64] after the GEMM
+    uint32_t* dig = tmw;                                 // [L][65] over the limb sums
     const uint32_t f0 = bx * kCrtCoefs;
     const uint32_t* pa = planes + size_t(prod) * 2 * P * t.N + f0;
     {
@@ -617,49 +632,66 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
 
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t row = lane & 31, kh = lane >> 5;
+    const uint32_t MT = t.Mcrt / 32;
     const v4i bias = {int(0x80808080u), int(0x80808080u), int(0x80808080u), int(0x80808080u)};
-    for (uint32_t mt = wave; mt < t.Mcrt / 32; mt += 4) {
-        const int8_t* wrow = t.wcrt + size_t(mt * 32 + row) * t.Kcrt + 16 * kh;
-        v4i A[KS];
+    v16i acc0[KT], acc1[KT];
 #pragma unroll
-        for (int k = 0; k < KS; ++k) A[k] = *reinterpret_cast<const v4i*>(wrow + 32 * k);
-        v16i acc0 = v16i{}, acc1 = v16i{};
+    for (int i = 0; i < KT; ++i) {
+        acc0[i] = v16i{};
+        acc1[i] = v16i{};
+        const uint32_t mt = wave + 4 * i;
+        if (mt < MT) {
+            const int8_t* wrow = t.wcrt + size_t(mt * 32 + row) * t.Kcrt + 16 * kh;
+            v4i A[KS];
 #pragma unroll
-        for (int k = 0; k < KS; ++k) {
-            const v4i B0 = *reinterpret_cast<const v4i*>(tmat + row * tstride + 32 * k + 16 * kh) ^ bias;
-            const v4i B1 = *reinterpret_cast<const v4i*>(tmat + (32 + row) * tstride + 32 * k + 16 * kh) ^ bias;
-            acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B1, acc1, 0, 0, 0);
+            for (int k = 0; k < KS; ++k) A[k] = *reinterpret_cast<const v4i*>(wrow + 32 * k);
+#pragma unroll
+            for (int k = 0; k < KS; ++k) {
+                const v4i B0 = *reinterpret_cast<const v4i*>(tmat + row * tstride + 32 * k + 16 * kh) ^ bias;
+                const v4i B1 =
+                    *reinterpret_cast<const v4i*>(tmat + (32 + row) * tstride + 32 * k + 16 * kh) ^ bias;
+                acc0[i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B0, acc0[i], 0, 0, 0);
+                acc1[i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B1, acc1[i], 0, 0, 0);
+            }
         }
+    }
+    __syncthreads();  // the term rows are dead from here on
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const uint32_t limb = mt * 8 + 2 * g + kh;
-            if (limb <= L) {
-                const int64_t cr = t.corr[limb];
-                lo[limb * kCrtCoefs + row] = int64_t(acc0[4 * g]) + int64_t(acc0[4 * g + 1]) * 256 +
-                                             int64_t(acc0[4 * g + 2]) * 65536 +
-                                             int64_t(acc0[4 * g + 3]) * 16777216 + cr;
-                lo[limb * kCrtCoefs + 32 + row] = int64_t(acc1[4 * g]) + int64_t(acc1[4 * g + 1]) * 256 +
-                                                  int64_t(acc1[4 * g + 2]) * 65536 +
-                                                  int64_t(acc1[4 * g + 3]) * 16777216 + cr;
+    for (int i = 0; i < KT; ++i) {
+        const uint32_t mt = wave + 4 * i;
+        if (mt < MT) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const uint32_t limb = mt * 8 + 2 * g + kh;
+                if (limb <= L) {
+                    const int64_t cr = t.corr[limb];
+                    lo[limb * kCrtCoefs + row] = int64_t(acc0[i][4 * g]) + int64_t(acc0[i][4 * g + 1]) * 256 +
+                                                 int64_t(acc0[i][4 * g + 2]) * 65536 +
+                                                 int64_t(acc0[i][4 * g + 3]) * 16777216 + cr;
+                    lo[limb * kCrtCoefs + 32 + row] =
+                        int64_t(acc1[i][4 * g]) + int64_t(acc1[i][4 * g + 1]) * 256 +
+                        int64_t(acc1[i][4 * g + 2]) * 65536 + int64_t(acc1[i][4 * g + 3]) * 16777216 + cr;
+                }
             }
         }
     }
     __syncthreads();
 
     if (threadIdx.x < kCrtCoefs) {
+        // One wave: lanes step through the limbs together, so digit i is
+        // written only after every lane has read limb sums 0..i.
         const uint32_t c = threadIdx.x;
-        uint32_t* d = dig + c * DS;
+        uint32_t* d = dig + c;  // digit i at d[i * kCrtDigitStride]
         int64_t carry = 0;
         for (uint32_t i = 0; i < L; ++i) {
             const int64_t tot = carry + lo[i * kCrtCoefs + c];
-            d[i] = uint32_t(tot);
+            d[i * kCrtDigitStride] = uint32_t(tot);
             carry = tot >> 32;
         }
         // z = sum d_i 2^(32 i) + top 2^(32 L), 0 <= z < 2^(W+18)
         int64_t top = carry + lo[L * kCrtCoefs + c];
         auto word = [&](uint32_t i) -> uint64_t {
-            return i < L ? uint64_t(d[i]) : (i == L ? uint64_t(top) : 0);
+            return i < L ? uint64_t(d[i * kCrtDigitStride]) : (i == L ? uint64_t(top) : 0);
         };
         const uint32_t a = t.qshift >> 5, bsh = t.qshift & 31;
         uint64_t zt = (word(a) | (word(a + 1) << 32)) >> bsh;  // z >> (W-32) < 2^50
@@ -673,17 +705,17 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
         for (uint32_t i = 0; i < L; ++i) {
             const uint64_t pq = uint64_t(Q) * t.q[i] + mc;
             mc = pq >> 32;
-            const int64_t dd = int64_t(d[i]) - int64_t(uint32_t(pq)) - br;
+            const int64_t dd = int64_t(d[i * kCrtDigitStride]) - int64_t(uint32_t(pq)) - br;
             br = dd < 0;
-            d[i] = uint32_t(dd);
+            d[i * kCrtDigitStride] = uint32_t(dd);
         }
         top -= int64_t(mc) + br;
         bool ge = top > 0;
         if (top == 0) {
             ge = true;
             for (int i = int(L) - 1; i >= 0; --i) {
-                if (d[i] != t.q[i]) {
-                    ge = d[i] > t.q[i];
+                if (d[i * kCrtDigitStride] != t.q[i]) {
+                    ge = d[i * kCrtDigitStride] > t.q[i];
                     break;
                 }
             }
@@ -691,9 +723,9 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
         if (ge) {
             int64_t b2 = 0;
             for (uint32_t i = 0; i < L; ++i) {
-                const int64_t dd = int64_t(d[i]) - int64_t(t.q[i]) - b2;
+                const int64_t dd = int64_t(d[i * kCrtDigitStride]) - int64_t(t.q[i]) - b2;
                 b2 = dd < 0;
-                d[i] = uint32_t(dd);
+                d[i * kCrtDigitStride] = uint32_t(dd);
             }
         }
     }
@@ -704,33 +736,49 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
         for (uint32_t cc = threadIdx.x >> 6; cc < kCrtCoefs; cc += kCrtThreads / 64) {
             const uint32_t f = f0 + cc;
             const uint32_t n = (f >> t.n1) + t.N2 * (f & (t.N1 - 1));
-            dst[size_t(n) * L + w] = dig[cc * DS + w];
+            dst[size_t(n) * L + w] = dig[w * kCrtDigitStride + cc];
         }
     }
 }
 
-// CRT with the k-step count (Kcrt / 32, 1..17) as a compile-time constant.
+// CRT with the k-step count (Kcrt / 32, 1..17) and the row tiles per wave as
+// compile-time constants.
+template <int KS>
+__device__ __forceinline__ void crt_tiles(const DevTables& t, const uint32_t* __restrict__ planes,
+                                          uint32_t* __restrict__ out, uint32_t bx, uint32_t prod, uint8_t* smem) {
+    if constexpr (KS <= 8) {
+        crt_wg<KS, 1>(t, planes, out, bx, prod, smem);
+    } else if constexpr (KS == 9) {
+        if (t.Mcrt / 32 <= 4)
+            crt_wg<KS, 1>(t, planes, out, bx, prod, smem);
+        else
+            crt_wg<KS, 2>(t, planes, out, bx, prod, smem);
+    } else {
+        crt_wg<KS, 2>(t, planes, out, bx, prod, smem);
+    }
+}
+
 __device__ __forceinline__ void crt_dispatch(const DevTables& t, const uint32_t* __restrict__ planes,
                                              uint32_t* __restrict__ out, uint32_t bx, uint32_t prod,
                                              uint8_t* smem) {
     switch (t.Kcrt / 32) {
-        case 1: crt_wg<1>(t, planes, out, bx, prod, smem); break;
-        case 2: crt_wg<2>(t, planes, out, bx, prod, smem); break;
-        case 3: crt_wg<3>(t, planes, out, bx, prod, smem); break;
-        case 4: crt_wg<4>(t, planes, out, bx, prod, smem); break;
-        case 5: crt_wg<5>(t, planes, out, bx, prod, smem); break;
-        case 6: crt_wg<6>(t, planes, out, bx, prod, smem); break;
-        case 7: crt_wg<7>(t, planes, out, bx, prod, smem); break;
-        case 8: crt_wg<8>(t, planes, out, bx, prod, smem); break;
-        case 9: crt_wg<9>(t, planes, out, bx, prod, smem); break;
-        case 10: crt_wg<10>(t, planes, out, bx, prod, smem); break;
-        case 11: crt_wg<11>(t, planes, out, bx, prod, smem); break;
-        case 12: crt_wg<12>(t, planes, out, bx, prod, smem); break;
-        case 13: crt_wg<13>(t, planes, out, bx, prod, smem); break;
-        case 14: crt_wg<14>(t, planes, out, bx, prod, smem); break;
-        case 15: crt_wg<15>(t, planes, out, bx, prod, smem); break;
-        case 16: crt_wg<16>(t, planes, out, bx, prod, smem); break;
-        default: crt_wg<17>(t, planes, out, bx, prod, smem); break;
+        case 1: crt_tiles<1>(t, planes, out, bx, prod, smem); break;
+        case 2: crt_tiles<2>(t, planes, out, bx, prod, smem); break;
+        case 3: crt_tiles<3>(t, planes, out, bx, prod, smem); break;
+        case 4: crt_tiles<4>(t, planes, out, bx, prod, smem); break;
+        case 5: crt_tiles<5>(t, planes, out, bx, prod, smem); break;
+        case 6: crt_tiles<6>(t, planes, out, bx, prod, smem); break;
+        case 7: crt_tiles<7>(t, planes, out, bx, prod, smem); break;
+        case 8: crt_tiles<8>(t, planes, out, bx, prod, smem); break;
+        case 9: crt_tiles<9>(t, planes, out, bx, prod, smem); break;
+        case 10: crt_tiles<10>(t, planes, out, bx, prod, smem); break;
+        case 11: crt_tiles<11>(t, planes, out, bx, prod, smem); break;
+        case 12: crt_tiles<12>(t, planes, out, bx, prod, smem); break;
+        case 13: crt_tiles<13>(t, planes, out, bx, prod, smem); break;
+        case 14: crt_tiles<14>(t, planes, out, bx, prod, smem); break;
+        case 15: crt_tiles<15>(t, planes, out, bx, prod, smem); break;
+        case 16: crt_tiles<16>(t, planes, out, bx, prod, smem); break;
+        default: crt_tiles<17>(t, planes, out, bx, prod, smem); break;
     }
 }
 
@@ -824,7 +872,7 @@ __global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __re
     middle_wg(t, planes, x, y, z, reinterpret_cast<double*>(smem_mid));
 }
 
-__global__ __launch_bounds__(kCrtThreads) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
+__global__ __launch_bounds__(kCrtThreads, 4) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
                                                      uint32_t* __restrict__ out) {
     extern __shared__ __align__(16) uint8_t smem_crt[];
     uint32_t x, y, z;
@@ -918,7 +966,7 @@ __global__ __launch_bounds__(kThreads) void k_inv_cols_groups(const GroupTable g
     }
 }
 
-__global__ __launch_bounds__(kCrtThreads) void k_crt_groups(const GroupTable g) {
+__global__ __launch_bounds__(kCrtThreads, 4) void k_crt_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_crt_g[];
     uint32_t x, y, z;
     uint32_t k;
@@ -932,6 +980,7 @@ bool shapes_ok(const DevTables& t, const BatchArgs& args) {
            t.N == t.N1 * t.N2 && t.N % kEncCoefs == 0 && t.Kenc % 32 == 0 && t.Kenc <= 256 &&
            t.Kcrt % 32 == 0 && t.Mcrt % 32 == 0 && t.P >= 1 && t.P <= kMaxPrimesCrt && t.L >= 3 &&
            t.L <= 64 && t.Kcrt >= 4 * (t.P + 1) && t.Mcrt >= 4 * t.L + 1 && t.Kenc >= 4 * t.L &&
+           t.Kcrt <= 32 * 17 && t.Mcrt <= 32 * 4 * crt_tiles_per_wave(t.Kcrt / 32, t.Mcrt / 32) &&
            t.qshift + 32 == t.W;
 }
 
