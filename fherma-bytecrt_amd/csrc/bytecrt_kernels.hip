@@ -551,9 +551,16 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
 // exact value is within 1/4 of an integer; the FP64 error is below 2^-37).  Then
 //   z = sum_{j<=P, a<4} byte_a(t_j) * W[j,a],  W[j,a] = 256^a w_j mod q,
 // as an int8 MFMA GEMM over balanced signed weight digits and biased bytes
-// (wave w computes digit-row tiles w and w+4), a signed carry chain per
-// coefficient, and r = z - Q q with Q = floor(z_hi / (q_hi + 1)) (exact),
-// which is floor(z/q) or one less, so one conditional subtraction finishes.
+// (wave w computes digit-row tiles w and w+4; the terms sit in LDS already
+// biased, byte ^ 0x80).  Every dot product is below 2^23 in magnitude (at most
+// 4 (P+1) <= 516 products of two values in [-128, 127]), so the int64 sum of a
+// limb's four digit rows, 2^24 a3 + 2^16 a2 + 2^8 a1 + a0 plus the bias
+// correction, stays below 2^49 and is formed exactly in FP64.  The quotient
+// Q = floor(z / q) is estimated before any carry from the top four limb sums
+// (truncation error below 2^-48 q), and one pass of exact FP64 limb steps
+//   t = s_i - Q q_i + carry,  carry = floor(t / 2^32),  digit = t - carry 2^32
+// (|t| < 2^52) yields z - Q q; a wrong estimate (z / q within 2^-40 of an
+// integer) leaves it in [-q, 2q), which one rare add or subtract of q fixes.
 //
 // One LDS region serves three phases: the term rows (GEMM operand), then the
 // int64 limb sums [L+1][64] (written once every wave has left the GEMM; the
@@ -568,6 +575,8 @@ constexpr uint32_t kCrtThreads = 256;
 constexpr uint32_t kCrtDigitStride = kCrtCoefs + 1;
 
 size_t crt_smem_bytes(const DevTables& t) {
+    // term rows (their last 16 bytes hold the waves' partial alpha sums), later
+    // the limb sums
     return std::max(size_t(kCrtCoefs) * (t.Kcrt + 16), size_t(t.L + 1) * kCrtCoefs * 8);
 }
 
@@ -575,6 +584,13 @@ size_t crt_smem_bytes(const DevTables& t) {
 // are at most 4 when Kcrt / 32 <= 8 and at most 8 otherwise.
 __host__ __device__ constexpr uint32_t crt_tiles_per_wave(uint32_t ks, uint32_t mt) {
     return ks <= 8 ? 1 : (ks == 9 && mt <= 4 ? 1 : 2);
+}
+
+// Sum of the four digit rows 4 g .. 4 g + 3 of an MFMA accumulator, weighted by
+// 256^row, exactly in FP64 (|a| < 2^23 keeps every partial sum below 2^48).
+__device__ __forceinline__ double limb_sum(const v16i& a, int g) {
+    return fma(double(a[4 * g + 3]), 16777216.0,
+               fma(double(a[4 * g + 2]), 65536.0, fma(double(a[4 * g + 1]), 256.0, double(a[4 * g]))));
 }
 
 // CRT for 64 coefficients with the k-loop of the GEMM (Kcrt / 32 steps) and the
@@ -589,15 +605,17 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
     const uint32_t groups = t.Kcrt / 16;   // 4-term groups per row
     uint8_t* tmat = crt_smem;
     uint32_t* tmw = reinterpret_cast<uint32_t*>(crt_smem);
-    int64_t* lo = reinterpret_cast<int64_t*>(crt_smem);  // [L+1][64] after the GEMM
+    double* lo = reinterpret_cast<double*>(crt_smem);    // [L+1][64] after the GEMM
     uint32_t* dig = tmw;                                 // [L][65] over the limb sums
     const uint32_t f0 = bx * kCrtCoefs;
     const uint32_t* pa = planes + size_t(prod) * 2 * P * t.N + f0;
     {
         // lane c gathers four consecutive terms of its coefficient (coalesced per
-        // plane); every load of a thread is issued before its LDS stores.
+        // plane); every load of a thread is issued before its LDS stores.  The
+        // wave's share of alpha comes straight from these registers (the term
+        // indices, hence the 1/p_j loads, are uniform across the wave).
         constexpr uint32_t kIters = (KS * 2 + 3) / 4;
-        const uint32_t c = threadIdx.x & 63, g0 = threadIdx.x >> 6;
+        const uint32_t c = threadIdx.x & 63, g0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         uint32_t v[kIters][4];
 #pragma unroll
         for (uint32_t it = 0; it < kIters; ++it) {
@@ -614,26 +632,37 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
             if (g < groups) {
                 uint32_t w4[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) w4[k] = 4 * g + k < P ? v[it][k] : 0u;
+                for (int k = 0; k < 4; ++k) w4[k] = (4 * g + k < P ? v[it][k] : 0u) ^ 0x80808080u;
                 *reinterpret_cast<uint4*>(tmw + c * tw + 4 * g) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
             }
         }
+        double part = 0.0;
+#pragma unroll
+        for (uint32_t it = 0; it < kIters; ++it) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t j = 4 * (g0 + 4 * it) + k;
+                if (j < P) part += double(v[it][k]) * t.invp[j];
+            }
+        }
+        // FP32 is ample for a share below 2^7: the sum's distance to the integer
+        // below it is at most 2^-5.
+        reinterpret_cast<float*>(tmw + c * tw + t.Kcrt / 4)[g0] = float(part);
     }
     __syncthreads();
-    {
-        const uint32_t c = threadIdx.x >> 2, part = threadIdx.x & 3;
-        double s = 0.0;
-        for (uint32_t j = part; j < P; j += 4) s += double(tmw[c * tw + j]) * t.invp[j];
-        s += __shfl_xor(s, 1, 4);
-        s += __shfl_xor(s, 2, 4);
-        if (part == 0) tmw[c * tw + P] = uint32_t(s + 0.5);
+    if (threadIdx.x < kCrtCoefs) {
+        // alpha = round(sum_j t_j / p_j); the sum lies within 2^-5 above an
+        // integer, far beyond any FP64 rounding, so the order of terms is free.
+        const uint32_t c = threadIdx.x;
+        const float* pr = reinterpret_cast<const float*>(tmw + c * tw + t.Kcrt / 4);
+        const double a = (double(pr[0]) + double(pr[1])) + (double(pr[2]) + double(pr[3]));
+        tmw[c * tw + P] = uint32_t(a + 0.5) ^ 0x80808080u;
     }
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t row = lane & 31, kh = lane >> 5;
     const uint32_t MT = t.Mcrt / 32;
-    const v4i bias = {int(0x80808080u), int(0x80808080u), int(0x80808080u), int(0x80808080u)};
     v16i acc0[KT], acc1[KT];
 #pragma unroll
     for (int i = 0; i < KT; ++i) {
@@ -647,9 +676,8 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
             for (int k = 0; k < KS; ++k) A[k] = *reinterpret_cast<const v4i*>(wrow + 32 * k);
 #pragma unroll
             for (int k = 0; k < KS; ++k) {
-                const v4i B0 = *reinterpret_cast<const v4i*>(tmat + row * tstride + 32 * k + 16 * kh) ^ bias;
-                const v4i B1 =
-                    *reinterpret_cast<const v4i*>(tmat + (32 + row) * tstride + 32 * k + 16 * kh) ^ bias;
+                const v4i B0 = *reinterpret_cast<const v4i*>(tmat + row * tstride + 32 * k + 16 * kh);
+                const v4i B1 = *reinterpret_cast<const v4i*>(tmat + (32 + row) * tstride + 32 * k + 16 * kh);
                 acc0[i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B0, acc0[i], 0, 0, 0);
                 acc1[i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B1, acc1[i], 0, 0, 0);
             }
@@ -664,13 +692,9 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
             for (int g = 0; g < 4; ++g) {
                 const uint32_t limb = mt * 8 + 2 * g + kh;
                 if (limb <= L) {
-                    const int64_t cr = t.corr[limb];
-                    lo[limb * kCrtCoefs + row] = int64_t(acc0[i][4 * g]) + int64_t(acc0[i][4 * g + 1]) * 256 +
-                                                 int64_t(acc0[i][4 * g + 2]) * 65536 +
-                                                 int64_t(acc0[i][4 * g + 3]) * 16777216 + cr;
-                    lo[limb * kCrtCoefs + 32 + row] =
-                        int64_t(acc1[i][4 * g]) + int64_t(acc1[i][4 * g + 1]) * 256 +
-                        int64_t(acc1[i][4 * g + 2]) * 65536 + int64_t(acc1[i][4 * g + 3]) * 16777216 + cr;
+                    const double cr = double(t.corr[limb]);
+                    lo[limb * kCrtCoefs + row] = limb_sum(acc0[i], g) + cr;
+                    lo[limb * kCrtCoefs + 32 + row] = limb_sum(acc1[i], g) + cr;
                 }
             }
         }
@@ -682,49 +706,50 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
         // written only after every lane has read limb sums 0..i.
         const uint32_t c = threadIdx.x;
         uint32_t* d = dig + c;  // digit i at d[i * kCrtDigitStride]
-        int64_t carry = 0;
-        for (uint32_t i = 0; i < L; ++i) {
-            const int64_t tot = carry + lo[i * kCrtCoefs + c];
-            d[i * kCrtDigitStride] = uint32_t(tot);
-            carry = tot >> 32;
+        const double* s = lo + c;  // limb sum i at s[i * kCrtCoefs]
+        // Q from the top four limb sums and limbs of q (units of 2^(32 (L-3)))
+        const uint32_t b = L - 3;
+        const double zt = ((s[L * kCrtCoefs] * 4294967296.0 + s[(L - 1) * kCrtCoefs]) * 4294967296.0 +
+                           s[(L - 2) * kCrtCoefs]) * 4294967296.0 + s[b * kCrtCoefs];
+        const double qe = (double(t.q[L - 1]) * 4294967296.0 + double(t.q[L - 2])) * 4294967296.0 + double(t.q[b]);
+        const double Q = fmax(floor(zt / qe), 0.0);
+        const uint32_t qv = t.q[min(c, L - 1)];  // lane i holds q_i (L <= 64)
+        double carry = 0.0;
+        for (uint32_t i0 = 0; i0 < L; i0 += 8) {
+            // eight limb sums in flight (index clamped to L: every read in bounds
+            // and not yet overwritten by a digit)
+            double sv[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) sv[k] = s[min(i0 + k, L) * kCrtCoefs];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) {
+                const uint32_t i = i0 + k;
+                if (i < L) {
+                    const double qi = double(uint32_t(__builtin_amdgcn_readlane(int(qv), int(i))));
+                    const double tt = fma(-Q, qi, sv[k]) + carry;
+                    carry = floor(tt * 0x1p-32);
+                    d[i * kCrtDigitStride] = uint32_t(fma(-carry, 4294967296.0, tt));
+                }
+            }
         }
-        // z = sum d_i 2^(32 i) + top 2^(32 L), 0 <= z < 2^(W+18)
-        int64_t top = carry + lo[L * kCrtCoefs + c];
-        auto word = [&](uint32_t i) -> uint64_t {
-            return i < L ? uint64_t(d[i * kCrtDigitStride]) : (i == L ? uint64_t(top) : 0);
-        };
-        const uint32_t a = t.qshift >> 5, bsh = t.qshift & 31;
-        uint64_t zt = (word(a) | (word(a + 1) << 32)) >> bsh;  // z >> (W-32) < 2^50
-        if (bsh) zt |= word(a + 2) << (64 - bsh);
-        const int64_t qt1 = int64_t(t.qt1);
-        int64_t Q = int64_t(double(zt) * t.inv_qt1);
-        const int64_t rem = int64_t(zt) - Q * qt1;
-        Q += (rem >= qt1) - (rem < 0);
-        uint64_t mc = 0;
-        int64_t br = 0;
-        for (uint32_t i = 0; i < L; ++i) {
-            const uint64_t pq = uint64_t(Q) * t.q[i] + mc;
-            mc = pq >> 32;
-            const int64_t dd = int64_t(d[i * kCrtDigitStride]) - int64_t(uint32_t(pq)) - br;
-            br = dd < 0;
-            d[i * kCrtDigitStride] = uint32_t(dd);
-        }
-        top -= int64_t(mc) + br;
-        bool ge = top > 0;
-        if (top == 0) {
-            ge = true;
+        // r = z - Q q = sum d_i 2^(32 i) + top 2^(32 L), -q <= r < 2q
+        const double top = s[L * kCrtCoefs] + carry;
+        bool sub = top > 0.0, add = top < 0.0;
+        if (top == 0.0) {
+            sub = true;  // r == q unless a limb differs
             for (int i = int(L) - 1; i >= 0; --i) {
-                if (d[i * kCrtDigitStride] != t.q[i]) {
-                    ge = d[i * kCrtDigitStride] > t.q[i];
+                const uint32_t di = d[i * kCrtDigitStride];
+                if (di != t.q[i]) {
+                    sub = di > t.q[i];
                     break;
                 }
             }
         }
-        if (ge) {
-            int64_t b2 = 0;
+        if (sub || add) {
+            int64_t cy = 0;
             for (uint32_t i = 0; i < L; ++i) {
-                const int64_t dd = int64_t(d[i * kCrtDigitStride]) - int64_t(t.q[i]) - b2;
-                b2 = dd < 0;
+                const int64_t dd = int64_t(d[i * kCrtDigitStride]) + (add ? int64_t(t.q[i]) : -int64_t(t.q[i])) + cy;
+                cy = dd >> 32;
                 d[i * kCrtDigitStride] = uint32_t(dd);
             }
         }
