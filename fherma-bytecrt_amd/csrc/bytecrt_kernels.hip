@@ -254,11 +254,22 @@ __device__ __forceinline__ void enc_compute(const DevTables& t, uint32_t* __rest
     // data fragments stay in registers) and weight tiles pt = w >> 1, +2, ...;
     // the two sub-tiles accumulate in independent chains.
     const uint32_t c0 = 2 * (wave & 1);
-    for (uint32_t pt = wave >> 1; pt < t.PTenc; pt += 2) {
+    // weight fragments of the next tile are loaded while this one computes
+    v4i A[KS];
+    {
+        const uint32_t pt = min(wave >> 1, t.PTenc - 1);
         const int8_t* wrow = t.wenc + (size_t(pt) * 32 + row) * t.Kenc + 16 * kh;
-        v4i A[KS];
 #pragma unroll
         for (int s2 = 0; s2 < KS; ++s2) A[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
+    }
+    for (uint32_t pt = wave >> 1; pt < t.PTenc; pt += 2) {
+        v4i An[KS];
+        {
+            const uint32_t pn = min(pt + 2, t.PTenc - 1);  // clamped: the last tile reloads itself
+            const int8_t* wrow = t.wenc + (size_t(pn) * 32 + row) * t.Kenc + 16 * kh;
+#pragma unroll
+            for (int s2 = 0; s2 < KS; ++s2) An[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
+        }
         // constants of this lane's four primes j = 8 pt + 2 g + kh (clamped;
         // the stores of primes past P are masked)
         double off[4], inv[4], pd[4];
@@ -302,6 +313,8 @@ __device__ __forceinline__ void enc_compute(const DevTables& t, uint32_t* __rest
                 if (j < t.P) out[j * t.N + f] = min(r, r - uint32_t(pd[g]));
             }
         }
+#pragma unroll
+        for (int s2 = 0; s2 < KS; ++s2) A[s2] = An[s2];
     }
 }
 
@@ -865,7 +878,7 @@ __device__ __forceinline__ void single_xyz(const DevTables& t, int stage, uint32
 // groups whose tiles are numbered group after group, so one launch per stage
 // covers products of different geometries and moduli.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_encode(DevTables t, const uint32_t* __restrict__ a,
+__global__ __launch_bounds__(kThreads, 3) void k_encode(DevTables t, const uint32_t* __restrict__ a,
                                                      const uint32_t* __restrict__ b,
                                                      uint32_t* __restrict__ planes) {
     extern __shared__ __align__(16) uint8_t smem_enc[];
@@ -897,7 +910,7 @@ __global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __re
     middle_wg(t, planes, x, y, z, reinterpret_cast<double*>(smem_mid));
 }
 
-__global__ __launch_bounds__(kCrtThreads, 4) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
+__global__ __launch_bounds__(kCrtThreads, 5) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
                                                      uint32_t* __restrict__ out) {
     extern __shared__ __align__(16) uint8_t smem_crt[];
     uint32_t x, y, z;
@@ -928,7 +941,7 @@ __device__ __forceinline__ bool locate(const GroupTable& g, int stage, uint32_t*
 // KS / NLOG1 = 0: any mix of shapes (switch per workgroup); otherwise every
 // group of the launch has that shape and the code is specialised for it.
 template <int KS>
-__global__ __launch_bounds__(kThreads) void k_encode_groups(const GroupTable g) {
+__global__ __launch_bounds__(kThreads, 3) void k_encode_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_enc_g[];
     uint32_t x, y, z;
     uint32_t k;
@@ -991,7 +1004,7 @@ __global__ __launch_bounds__(kThreads) void k_inv_cols_groups(const GroupTable g
     }
 }
 
-__global__ __launch_bounds__(kCrtThreads, 4) void k_crt_groups(const GroupTable g) {
+__global__ __launch_bounds__(kCrtThreads, 5) void k_crt_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_crt_g[];
     uint32_t x, y, z;
     uint32_t k;
