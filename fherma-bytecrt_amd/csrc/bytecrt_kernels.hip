@@ -102,30 +102,6 @@ __device__ __forceinline__ void ct_round(double (&v)[16], const double* __restri
     }
 }
 
-// ct_round on two operands of the same prime and block at once: each twiddle is
-// loaded once and the two butterfly chains are independent.
-template <int K>
-__device__ __forceinline__ void ct_round2(double (&v)[16], double (&u)[16], const double* __restrict__ z,
-                                          uint32_t mult, uint32_t e0, uint32_t o0, double p, double pinv) {
-#pragma unroll
-    for (int l = 0; l < K; ++l) {
-        const int half = 1 << (K - 1 - l);
-#pragma unroll
-        for (int sb = 0; sb < (16 >> K); ++sb) {
-#pragma unroll
-            for (int g = 0; g < (1 << l); ++g) {
-                const double w = z[(mult << (e0 + l)) + ((o0 + sb) << l) + g];
-#pragma unroll
-                for (int x = 0; x < half; ++x) {
-                    const int i0 = (sb << K) + g * 2 * half + x;
-                    ct_bfly(v[i0], v[i0 + half], w, p, pinv);
-                    ct_bfly(u[i0], u[i0 + half], w, p, pinv);
-                }
-            }
-        }
-    }
-}
-
 template <int K>
 __device__ __forceinline__ void gs_round(double (&v)[16], const double* __restrict__ z, uint32_t mult,
                                          uint32_t e0, uint32_t o0, double p, double pinv) {
@@ -445,12 +421,59 @@ __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __rest
     for (int x = 0; x < 16; ++x) col[tau + T * x] = canon(mul_mod(v[x], sc2, p, pinv), p, pinv);
 }
 
+// Twiddles of a radix-16 round (K = 4, one sub-block): tw[2^l - 1 + g] is the
+// twiddle of group g at local level l, loaded ahead of the round that uses it.
+__device__ __forceinline__ void load_tw16(double (&tw)[15], const double* __restrict__ z, uint32_t mult,
+                                          uint32_t e0, uint32_t o0) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+#pragma unroll
+        for (int g = 0; g < (1 << l); ++g) tw[(1 << l) - 1 + g] = z[(mult << (e0 + l)) + (o0 << l) + g];
+}
+
+// Forward radix-16 round on two operands of the same prime and block at once:
+// each twiddle serves two independent butterfly chains.
+__device__ __forceinline__ void ct16x2_tw(double (&v)[16], double (&u)[16], const double (&tw)[15], double p,
+                                          double pinv) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const int half = 1 << (3 - l);
+#pragma unroll
+        for (int g = 0; g < (1 << l); ++g) {
+            const double w = tw[(1 << l) - 1 + g];
+#pragma unroll
+            for (int x = 0; x < half; ++x) {
+                const int i0 = g * 2 * half + x;
+                ct_bfly(v[i0], v[i0 + half], w, p, pinv);
+                ct_bfly(u[i0], u[i0 + half], w, p, pinv);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void gs16_tw(double (&v)[16], const double (&tw)[15], double p, double pinv) {
+#pragma unroll
+    for (int l = 3; l >= 0; --l) {
+        const int half = 1 << (3 - l);
+#pragma unroll
+        for (int g = 0; g < (1 << l); ++g) {
+            const double w = tw[(1 << l) - 1 + g];
+#pragma unroll
+            for (int x = 0; x < half; ++x) {
+                const int i0 = g * 2 * half + x;
+                gs_bfly(v[i0], v[i0 + half], w, p, pinv);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Middle: 16 blocks (256 contiguous natural coefficients each) of one prime,
 // forward levels n1..n-1 for A and B, pointwise product, inverse levels back.
 // Grid (N1/16, batch, P): workgroups of one prime run together so its block
-// twiddles stay in L2 across the whole batch.  One FP64 exchange buffer is
-// reused for A, then B, then the product.
+// twiddles stay in L2 across the whole batch; each radix-16 round's 15
+// twiddles are loaded while the round before it computes.  One exchange
+// buffer holds A and B (u32), then the product (FP64).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMidS = exchange_stride<256>();
 
@@ -492,6 +515,8 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
     uint32_t* rb = ub + bl * kMidS;
     double* rx = sx + bl * kMidS;
     double v[16], w[16];
+    double twa[15], twb[15];
+    load_tw16(twa, zf, mult, 0, 0);
     {
         uint4 qa[4], qb[4];
         mid_fetch(qa, pa, t.N1, b0);
@@ -516,7 +541,8 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
         v[x] = double(ra[skew(tau + 16 * x)]);
         w[x] = double(rb[skew(tau + 16 * x)]);
     }
-    ct_round2<4>(v, w, zf, mult, 0, 0, p, pinv);
+    load_tw16(twb, zf, mult, 4, tau);
+    ct16x2_tw(v, w, twa, p, pinv);
 #pragma unroll
     for (int x = 0; x < 16; ++x) {
         ra[skew(tau + 16 * x)] = canon(v[x], p, pinv);
@@ -529,18 +555,20 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
         v[x] = double(ra[skew(16 * tau + x)]);
         w[x] = double(rb[skew(16 * tau + x)]);
     }
-    ct_round2<4>(v, w, zf, mult, 4, tau, p, pinv);
+    load_tw16(twa, zi, mult, 4, tau);
+    ct16x2_tw(v, w, twb, p, pinv);
     // pointwise product (|v|, |w| < 2^34) and inverse sub-levels 7..4
 #pragma unroll
     for (int x = 0; x < 16; ++x) v[x] = mul_mod(v[x], w[x], p, pinv);
-    gs_round<4>(v, zi, mult, 4, tau, p, pinv);
+    load_tw16(twb, zi, mult, 0, 0);
+    gs16_tw(v, twa, p, pinv);
     __syncthreads();
 #pragma unroll
     for (int x = 0; x < 16; ++x) rx[skew(16 * tau + x)] = v[x];
     __syncthreads();
 #pragma unroll
     for (int x = 0; x < 16; ++x) v[x] = rx[skew(tau + 16 * x)];
-    gs_round<4>(v, zi, mult, 0, 0, p, pinv);
+    gs16_tw(v, twb, p, pinv);
     __syncthreads();
     {
         // canonical residues back in place, then row u out as four 16-byte stores
