@@ -43,7 +43,9 @@ def parse():
     ap.add_argument("--only-stage", choices=["encode", "fwd_cols", "middle", "inv_cols", "crt", "product"],
                     help="profiling mode: run one stage (or the whole product) --reps times on the batch")
     ap.add_argument("--reps", type=int, default=5)
-    ap.add_argument("--sweep", action="store_true", help="time batch pipelining settings and exit")
+    ap.add_argument("--sweep", action="store_true",
+                    help="autotune the batch pipelining, write the pipeline table and exit")
+    ap.add_argument("--sweep-out", default="gpurun_out/pipeline_table.json")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="profiling mode: device-to-device copy of this many MiB, --reps times")
     return ap.parse_args()
@@ -363,36 +365,27 @@ def cpu_baseline(pkg, q, seconds: float):
 
 
 def sweep_mode(args, torch):
-    """Batch time for each (chunk, streams) pipelining setting."""
+    """Autotune the batch pipelining for the bench geometry (every candidate's
+    product is checked bit for bit against the first) and record the choice
+    in a pipeline table (copy it into the package to make it the default)."""
     sys.path.insert(0, str(ROOT))
     import __graft_entry__ as ge
     pkg = ge.load_package()
+    import bytecrt_amd.tune as tune
     q = pkg.datagen.modulus(W, 2026)
     eng = pkg.Engine(N, W, q, max_batch=B)
     a = device_operands(torch, B, N, q, 11)
     b = device_operands(torch, B, N, q, 12)
     c = torch.empty_like(a)
-    ref = torch.empty_like(a)
     torch.cuda.synchronize()
     stream = torch.cuda.Stream()
-    eng.set_pipeline(B, 1)
-    eng.run_device(a, b, ref, B, stream=stream)
-    torch.cuda.synchronize()
-    rows = []
-    for chunk, streams in [(128, 1), (64, 2), (32, 2), (32, 4), (16, 2), (16, 4), (16, 8), (8, 4), (8, 8)]:
-        eng.set_pipeline(chunk, streams)
-        for graph in (False, True):
-            for _ in range(2):
-                eng.run_device(a, b, c, B, stream=stream, graph=graph)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(5):
-                eng.run_device(a, b, c, B, stream=stream, graph=graph)
-            torch.cuda.synchronize()
-            ms = (time.perf_counter() - t0) * 1e3 / 5
-            rows.append({"chunk": chunk, "streams": streams, "graph": graph, "ms": round(ms, 3),
-                         "exact": bool(torch.equal(c, ref))})
-            print(json.dumps(rows[-1]), flush=True)
+    res = tune.tune_pipeline(eng, a, b, c, B, reps=args.reps, stream=stream)
+    for r in res["rows"]:
+        print(json.dumps(r), flush=True)
+    out = Path(args.sweep_out)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    tune.record(N, W, B, res, device=torch.cuda.get_device_name(0), path=out)
+    print(json.dumps({"best": res["best"], "table": str(out)}), flush=True)
 
 
 def main():
@@ -438,7 +431,9 @@ def main():
     # exactness spot check: batched product 7 equals its plain result
     eng.run_device(a[7], b[7], c1, 1, stream=stream)
 
-    # -- batched: one graph launch per step ---------------------------------
+    # -- batched: one graph launch per step, pipelining from the tuned table ---
+    tuned = eng.apply_tuning(B)
+    pipeline = {"chunk": eng.pipeline[0], "streams": eng.pipeline[1], "source": "table" if tuned else "default"}
     for _ in range(max(args.warmup, 1)):
         eng.run_device(a, b, c, B, stream=stream, graph=True)
     torch.cuda.synchronize()
@@ -519,6 +514,7 @@ def main():
                     "p50_ms": round(float(np.percentile(batch_ms, 50)), 4),
                     "p99_ms": round(float(np.percentile(batch_ms, 99)), 4)},
         "batched_vs_plain": round(value / plain_pps, 3),
+        "pipeline": pipeline,
         "exact_spot_check": exact,
         "stages": stages,
         "ntt_hbm_frac": round(ntt_gbs / HBM_PEAK_GBS, 4),

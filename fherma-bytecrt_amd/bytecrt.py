@@ -151,6 +151,7 @@ class Engine:
         if rc != BCRT_OK:
             _raise(rc, err.value.decode())
         self._h = handle
+        self.pipeline = (16, 4)  # the library's default (chunk, streams)
         g = np.zeros(10, dtype=np.uint32)
         lib.bcrt_geometry(self._h, g.ctypes.data_as(_u32p))
         (self.N, self.W, self.L, self.P, self.N1, self.N2, self.Kenc, self.Kcrt, self.Mcrt,
@@ -211,6 +212,13 @@ class Engine:
     def set_pipeline(self, chunk: int, streams: int):
         """Split batches above `chunk` products over `streams` worker streams."""
         self._check(load_library().bcrt_set_pipeline(self._h, chunk, streams))
+        self.pipeline = (chunk, streams)
+
+    def apply_tuning(self, batch: int, table: dict | None = None) -> bool:
+        """Use the pipelining the autotuner recorded for this geometry and batch
+        (tune.py's table), if any; returns whether one was found."""
+        from . import tune
+        return tune.apply(self, batch, table)
 
     def synchronize(self):
         self._check(load_library().bcrt_synchronize(self._h))

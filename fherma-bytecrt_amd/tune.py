@@ -1,0 +1,102 @@
+"""Pipeline autotuner for the device-resident batch path, cached per geometry.
+
+A batch above ``chunk`` products is split into chunks that run on ``streams``
+worker streams (``Engine.set_pipeline``), so the stages of different chunks
+overlap.  The best setting depends on the geometry (kernel sizes, how fully
+one chunk fills the GPU) and the batch.  ``tune_pipeline`` times every
+candidate on the caller's device operands with the hipGraph path the batched
+run uses (one eager run first, which also creates the worker streams outside
+any capture), checks each candidate's result bit for bit against the first,
+and leaves the engine on the fastest.  ``record`` / ``apply`` keep the choice
+in a JSON table keyed by (N, W, batch), so a later engine of the same
+geometry starts from the tuned setting without timing again.
+
+Timing uses HIP events on one stream (torch's), so this module needs torch;
+the library itself does not.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from pathlib import Path
+
+# (chunk, streams); chunk 0 means the whole batch as one chunk.
+DEFAULT_CANDIDATES = ((0, 1), (64, 2), (32, 2), (32, 4), (16, 2), (16, 4), (16, 8), (8, 4), (8, 8))
+
+TABLE = Path(__file__).resolve().parent / "pipeline_table.json"
+
+
+def _key(N: int, W: int, batch: int) -> str:
+    return f"N{N}_W{W}_B{batch}"
+
+
+def tune_pipeline(engine, a, b, c, batch: int, candidates=DEFAULT_CANDIDATES, reps: int = 5, stream=None):
+    """Time each (chunk, streams) candidate for ``batch`` products of device
+    operands a, b into c (all [batch][N][L] int32 tensors on the GPU).  Returns
+    {"best": {...}, "rows": [...]}; the engine is left on the best setting.
+    Raises if any candidate's product differs from the first candidate's."""
+    import torch
+
+    if batch < 1 or batch > engine.max_batch:
+        raise ValueError(f"ByteCRT batch must be in [1, max_batch={engine.max_batch}]")
+    stream = stream or torch.cuda.current_stream()
+    ref = None
+    rows = []
+    for chunk, streams in candidates:
+        chunk = batch if chunk == 0 else chunk
+        if chunk > batch or (chunk == batch and streams > 1):
+            continue
+        engine.set_pipeline(chunk, streams)
+        engine.run_device(a, b, c, batch, stream=stream)  # eager: creates worker streams
+        engine.run_device(a, b, c, batch, stream=stream, graph=True)  # capture + first replay
+        stream.synchronize()
+        if ref is None:
+            ref = c.clone()
+        elif not torch.equal(c, ref):
+            raise RuntimeError(f"ByteCRT pipeline ({chunk}, {streams}) changed the product")
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(reps):
+            engine.run_device(a, b, c, batch, stream=stream, graph=True)
+        t1.record(stream)
+        t1.synchronize()
+        rows.append({"chunk": chunk, "streams": streams, "ms": round(t0.elapsed_time(t1) / reps, 4)})
+    best = min(rows, key=lambda r: r["ms"])
+    engine.set_pipeline(best["chunk"], best["streams"])
+    return {"best": best, "rows": rows}
+
+
+def load_table(path: Path | None = None) -> dict:
+    path = Path(path or os.environ.get("BCRT_PIPELINE_TABLE", TABLE))
+    if not path.exists():
+        return {}
+    return json.loads(path.read_text())
+
+
+def record(N: int, W: int, batch: int, result: dict, device: str, path: Path | None = None) -> dict:
+    """Store a tune_pipeline result in the table (returns the updated table)."""
+    path = Path(path or os.environ.get("BCRT_PIPELINE_TABLE", TABLE))
+    table = load_table(path)
+    best = result["best"]
+    table[_key(N, W, batch)] = {"chunk": best["chunk"], "streams": best["streams"], "ms": best["ms"],
+                                "device": device, "date": time.strftime("%Y-%m-%d")}
+    path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
+    return table
+
+
+def lookup(N: int, W: int, batch: int, table: dict | None = None):
+    """The tuned (chunk, streams) for this geometry and batch, or None."""
+    table = load_table() if table is None else table
+    e = table.get(_key(N, W, batch))
+    return None if e is None else (int(e["chunk"]), int(e["streams"]))
+
+
+def apply(engine, batch: int, table: dict | None = None) -> bool:
+    """Put the engine on its tuned setting for ``batch`` products if the table
+    has one; returns whether it did."""
+    hit = lookup(engine.N, engine.W, batch, table)
+    if hit is None:
+        return False
+    engine.set_pipeline(*hit)
+    return True

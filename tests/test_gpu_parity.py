@@ -206,6 +206,38 @@ def test_autotune_keeps_results(pkg, gu):
         e.autotune(B + 1)
 
 
+def test_tune_pipeline_exact(pkg, gu, tmp_path):
+    """The autotuner times its candidates through the graph path, keeps every
+    result bit-exact, leaves the engine on the fastest, and a fresh engine
+    that applies the recorded table computes the same products."""
+    import importlib
+    import torch
+    tune = importlib.import_module("bytecrt_amd.tune")
+    N, W, B = 32768, 868, 12
+    q = pkg.datagen.modulus(W, 77)
+    a = np.stack([pkg.datagen.operands(N, q, 700 + i) for i in range(B)])
+    b = np.stack([pkg.datagen.operands(N, q, 800 + i) for i in range(B)])
+    e = pkg.Engine(N, W, q, max_batch=B)
+    da, db = gu.to_dev(a).view(B, N, -1), gu.to_dev(b).view(B, N, -1)
+    out = torch.empty_like(da)
+    e.set_pipeline(B, 1)
+    e.run_device(da, db, out, B)
+    gu.sync()
+    plain = out.clone()
+    res = tune.tune_pipeline(e, da, db, out, B, candidates=((0, 1), (8, 2), (4, 4), (4, 2)), reps=2)
+    assert {(r["chunk"], r["streams"]) for r in res["rows"]} == {(12, 1), (8, 2), (4, 4), (4, 2)}
+    assert e.pipeline == (res["best"]["chunk"], res["best"]["streams"])
+    assert torch.equal(out, plain)
+    path = tmp_path / "table.json"
+    tune.record(N, W, B, res, device="test", path=path)
+    f = pkg.Engine(N, W, q, max_batch=B)
+    assert f.apply_tuning(B, tune.load_table(path)) and f.pipeline == e.pipeline
+    out2 = torch.zeros_like(out)
+    f.run_device(da, db, out2, B, graph=True)
+    gu.sync()
+    assert torch.equal(out2, plain)
+
+
 def test_whole_call_multi_chunk(pkg, gu):
     """The host whole call at N=65,536 with B=5: three chunks through the
     staging ring, several pieces per copy, equal to the device-resident run."""

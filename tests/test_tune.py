@@ -1,0 +1,57 @@
+"""Pipeline autotuner table: record, look up and apply (CPU; the GPU timing
+itself is covered by test_gpu_parity.test_tune_pipeline_exact)."""
+import importlib.util
+import json
+from pathlib import Path
+
+import pytest
+
+TUNE = Path(__file__).resolve().parents[1] / "fherma-bytecrt_amd" / "tune.py"
+
+
+@pytest.fixture(scope="module")
+def tune():
+    spec = importlib.util.spec_from_file_location("bcrt_tune_cpu", TUNE)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+class FakeEngine:
+    def __init__(self, N, W, max_batch=128):
+        self.N, self.W, self.max_batch = N, W, max_batch
+        self.calls = []
+
+    def set_pipeline(self, chunk, streams):
+        self.calls.append((chunk, streams))
+
+
+def test_record_lookup_apply(tune, tmp_path):
+    path = tmp_path / "table.json"
+    res = {"best": {"chunk": 32, "streams": 4, "ms": 17.5}, "rows": []}
+    tune.record(65536, 1747, 128, res, device="test", path=path)
+    table = tune.load_table(path)
+    assert tune.lookup(65536, 1747, 128, table) == (32, 4)
+    assert tune.lookup(65536, 1747, 64, table) is None       # batch is part of the key
+    assert tune.lookup(32768, 868, 128, table) is None
+    e = FakeEngine(65536, 1747)
+    assert tune.apply(e, 128, table) and e.calls == [(32, 4)]
+    assert not tune.apply(e, 1, table) and e.calls == [(32, 4)]
+    # a second record keeps the first entry
+    tune.record(32768, 868, 12, {"best": {"chunk": 8, "streams": 2, "ms": 1.0}, "rows": []}, "test", path)
+    table = json.loads(path.read_text())
+    assert set(table) == {"N65536_W1747_B128", "N32768_W868_B12"}
+
+
+def test_table_from_environment(tune, tmp_path, monkeypatch):
+    path = tmp_path / "env.json"
+    path.write_text(json.dumps({"N8192_W220_B4": {"chunk": 2, "streams": 2, "ms": 0.1}}))
+    monkeypatch.setenv("BCRT_PIPELINE_TABLE", str(path))
+    assert tune.lookup(8192, 220, 4) == (2, 2)
+
+
+def test_shipped_table_is_valid(tune):
+    table = tune.load_table(tune.TABLE)
+    for key, e in table.items():
+        assert key.startswith("N") and "_W" in key and "_B" in key
+        assert 1 <= e["chunk"] and 1 <= e["streams"] <= 16
