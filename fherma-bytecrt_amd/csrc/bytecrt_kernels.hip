@@ -15,8 +15,10 @@
 // The NTT is the merged negacyclic Cooley-Tukey / Gentleman-Sande pair with
 // psi^{+-bitrev(k)} twiddles (no separate twist pass, bit-reversed spectrum
 // consumed in place).  Inside a kernel residues are FP64 integers in a lazy
-// balanced range and every modular product is exact (see mul_mod); between
-// kernels they are canonical u32 in [0, p).
+// balanced range and every modular product is exact (see mul_mod).  Between
+// kernels the encode output and the CRT terms are canonical u32 in [0, p);
+// the forward-column and middle outputs, read only by the next NTT kernel,
+// are balanced int32 (|r| <= p/2 + 1), which is cheaper to form.
 #include "bytecrt_kernels.hpp"
 
 #include <algorithm>
@@ -44,6 +46,7 @@ constexpr int kEncChains = 2;        // independent MFMA accumulator chains in e
 //   below 2^31 + 2^18, so the FMA computes it exactly, and the final sum is
 //   exact too.  Hence r == x w (mod p) exactly, with |r| <= p/2 + 2^-12 p.
 // canon(x): |x| <= 2^40 -> the representative in [0, p).
+// balanced(x): |x| <= 2^40 -> a representative with |r| <= p/2 + 1 (int32).
 // Forward butterflies grow |x| by at most 0.51 p per level, inverse (GS) sums
 // at most double it: eight levels from |x| < p stay below 2^39.
 // ---------------------------------------------------------------------------
@@ -62,6 +65,15 @@ __device__ __forceinline__ uint32_t canon(double x, double p, double pinv) {
     r = r < 0.0 ? r + p : r;
     return uint32_t(r);
 }
+
+__device__ __forceinline__ int32_t balanced(double x, double p, double pinv) {
+    const double q = fma(x, pinv, kRound) - kRound;
+    return int32_t(fma(-q, p, x));  // exact, |r| <= p/2 + 1
+}
+
+// A residue word of a plane or exchange buffer as an FP64 integer: canonical
+// u32 and balanced int32 words agree on [0, 2^31).
+__device__ __forceinline__ double word_value(uint32_t w) { return double(int32_t(w)); }
 
 // (x, y) -> (x + w y, x - w y)
 __device__ __forceinline__ void ct_bfly(double& x, double& y, double w, double p, double pinv) {
@@ -374,11 +386,11 @@ __device__ __forceinline__ void fwd_cols_wg(const DevTables& t, uint32_t* __rest
 #pragma unroll
         for (int x = 0; x < 16; x += 4)
             *reinterpret_cast<uint4*>(col + 16 * tau + x) =
-                make_uint4(canon(v[x], p, pinv), canon(v[x + 1], p, pinv), canon(v[x + 2], p, pinv),
-                           canon(v[x + 3], p, pinv));
+                make_uint4(uint32_t(balanced(v[x], p, pinv)), uint32_t(balanced(v[x + 1], p, pinv)),
+                           uint32_t(balanced(v[x + 2], p, pinv)), uint32_t(balanced(v[x + 3], p, pinv)));
     } else {
 #pragma unroll
-        for (int x = 0; x < 16; ++x) col[x] = canon(v[x], p, pinv);
+        for (int x = 0; x < 16; ++x) col[x] = uint32_t(balanced(v[x], p, pinv));
     }
 }
 
@@ -399,10 +411,10 @@ __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __rest
 #pragma unroll
         for (int x = 0; x < 16; x += 4) {
             const uint4 q4 = *reinterpret_cast<const uint4*>(col + 16 * tau + x);
-            v[x] = double(q4.x);
-            v[x + 1] = double(q4.y);
-            v[x + 2] = double(q4.z);
-            v[x + 3] = double(q4.w);
+            v[x] = word_value(q4.x);
+            v[x + 1] = word_value(q4.y);
+            v[x + 2] = word_value(q4.z);
+            v[x + 3] = word_value(q4.w);
         }
         gs_round<K2>(v, z, 1, 4, tau * (16 >> K2), p, pinv);
         double* sc = s + cw * S;
@@ -413,7 +425,7 @@ __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __rest
         for (int x = 0; x < 16; ++x) v[x] = sc[skew(tau + T * x)];
     } else {
 #pragma unroll
-        for (int x = 0; x < 16; ++x) v[x] = double(col[x]);
+        for (int x = 0; x < 16; ++x) v[x] = word_value(col[x]);
     }
     gs_round<4>(v, z, 1, 0, 0, p, pinv);
     const double sc2 = t.scale[j];
@@ -485,17 +497,6 @@ __device__ __forceinline__ void mid_fetch(uint4 (&q4)[4], const uint32_t* __rest
     for (int k = 0; k < 4; ++k) q4[k] = src[k];
 }
 
-__device__ __forceinline__ void mid_store_lds(double* __restrict__ dst, const uint4 (&q4)[4]) {
-    const uint32_t o = skew(threadIdx.x);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        dst[(4 * k + 0) * kMidS + o] = double(q4[k].x);
-        dst[(4 * k + 1) * kMidS + o] = double(q4[k].y);
-        dst[(4 * k + 2) * kMidS + o] = double(q4[k].z);
-        dst[(4 * k + 3) * kMidS + o] = double(q4[k].w);
-    }
-}
-
 __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restrict__ planes, uint32_t bx,
                                           uint32_t prod, uint32_t j, double* sx) {
     static_assert(kMidBlocks * kMidS * sizeof(double) <= kExchangeBytes, "exchange buffer");
@@ -507,7 +508,7 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
     const uint32_t mult = t.N1 + b0 + bl;
     const double* zf = t.zf + size_t(j) * t.N;
     const double* zi = t.zi + size_t(j) * t.N;
-    // The exchange buffer holds both operands as canonical u32 (A in the first
+    // The exchange buffer holds both operands as 32-bit residue words (A in the first
     // half, B in the second) for the forward sub-levels, then the product as FP64.
     uint32_t* ua = reinterpret_cast<uint32_t*>(sx);
     uint32_t* ub = ua + kMidBlocks * kMidS;
@@ -538,22 +539,22 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
     // forward sub-levels 0..3 (positions tau + 16x) of A and B together
 #pragma unroll
     for (int x = 0; x < 16; ++x) {
-        v[x] = double(ra[skew(tau + 16 * x)]);
-        w[x] = double(rb[skew(tau + 16 * x)]);
+        v[x] = word_value(ra[skew(tau + 16 * x)]);
+        w[x] = word_value(rb[skew(tau + 16 * x)]);
     }
     load_tw16(twb, zf, mult, 4, tau);
     ct16x2_tw(v, w, twa, p, pinv);
 #pragma unroll
     for (int x = 0; x < 16; ++x) {
-        ra[skew(tau + 16 * x)] = canon(v[x], p, pinv);
-        rb[skew(tau + 16 * x)] = canon(w[x], p, pinv);
+        ra[skew(tau + 16 * x)] = uint32_t(balanced(v[x], p, pinv));
+        rb[skew(tau + 16 * x)] = uint32_t(balanced(w[x], p, pinv));
     }
     __syncthreads();
     // sub-levels 4..7 (positions 16 tau + x)
 #pragma unroll
     for (int x = 0; x < 16; ++x) {
-        v[x] = double(ra[skew(16 * tau + x)]);
-        w[x] = double(rb[skew(16 * tau + x)]);
+        v[x] = word_value(ra[skew(16 * tau + x)]);
+        w[x] = word_value(rb[skew(16 * tau + x)]);
     }
     load_tw16(twa, zi, mult, 4, tau);
     ct16x2_tw(v, w, twb, p, pinv);
@@ -571,10 +572,10 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
     gs16_tw(v, twb, p, pinv);
     __syncthreads();
     {
-        // canonical residues back in place, then row u out as four 16-byte stores
+        // balanced residues back in place, then row u out as four 16-byte stores
         uint32_t* sy = reinterpret_cast<uint32_t*>(sx);
 #pragma unroll
-        for (int x = 0; x < 16; ++x) sy[bl * kMidS + skew(tau + 16 * x)] = canon(v[x], p, pinv);
+        for (int x = 0; x < 16; ++x) sy[bl * kMidS + skew(tau + 16 * x)] = uint32_t(balanced(v[x], p, pinv));
         __syncthreads();
         const uint32_t u = threadIdx.x;
         uint4* dst = reinterpret_cast<uint4*>(pa + size_t(u) * t.N1 + b0);
