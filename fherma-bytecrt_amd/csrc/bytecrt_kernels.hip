@@ -589,8 +589,10 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
 
 // ---------------------------------------------------------------------------
 // CRT: 64 coefficients (consecutive column-major positions), four waves.
-// Terms t_j are in operand-A planes.  alpha = round(sum t_j / p_j) in FP64 (the
-// exact value is within 1/4 of an integer; the FP64 error is below 2^-37).  Then
+// Terms t_j are in operand-A planes.  alpha = round(sum t_j / p_j): the plan's
+// guard P > 4 N (q-1)^2 puts the exact sum in [alpha, alpha + 1/4), and the
+// computed one (FP64 per-wave shares rounded to FP32, summed in FP64) is
+// within 2^-16 of it.  Then
 //   z = sum_{j<=P, a<4} byte_a(t_j) * W[j,a],  W[j,a] = 256^a w_j mod q,
 // as an int8 MFMA GEMM over balanced signed weight digits and biased bytes
 // (wave w computes digit-row tiles w and w+4; the terms sit in LDS already
@@ -687,14 +689,14 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
                 if (j < P) part += double(v[it][k]) * t.invp[j];
             }
         }
-        // FP32 is ample for a share below 2^7: the sum's distance to the integer
-        // below it is at most 2^-5.
+        // A share (at most 36 terms, each t_j / p_j < 1) is below 2^6, so FP32
+        // keeps it within 2^-18; the sum's margin to the next integer is 3/4.
         reinterpret_cast<float*>(tmw + c * tw + t.Kcrt / 4)[g0] = float(part);
     }
     __syncthreads();
     if (threadIdx.x < kCrtCoefs) {
-        // alpha = round(sum_j t_j / p_j); the sum lies within 2^-5 above an
-        // integer, far beyond any FP64 rounding, so the order of terms is free.
+        // alpha = round(sum_j t_j / p_j); the sum lies within 1/4 above an
+        // integer, far beyond the rounding of the shares, so their order is free.
         const uint32_t c = threadIdx.x;
         const float* pr = reinterpret_cast<const float*>(tmw + c * tw + t.Kcrt / 4);
         const double a = (double(pr[0]) + double(pr[1])) + (double(pr[2]) + double(pr[3]));
