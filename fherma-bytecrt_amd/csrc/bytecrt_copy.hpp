@@ -82,8 +82,12 @@ class CopyPool {
 // the page size changes; failure (no THP) is harmless.
 void advise_huge(void* p, size_t bytes);
 
-// Workers for one copy pool: half the hardware threads, less the caller, at most
-// 7; the environment variable BCRT_COPY_THREADS overrides the pool size.
+// Workers for one copy pool of the whole call: half the hardware threads, less
+// the caller, at most 7; BCRT_COPY_THREADS overrides the pool size.
 unsigned pool_workers();
+
+// Workers for one copy pool of the product server: all hardware threads but
+// two, less the caller; BCRT_SERVER_COPY_THREADS overrides the pool size.
+unsigned server_pool_workers();
 
 }  // namespace bcrt

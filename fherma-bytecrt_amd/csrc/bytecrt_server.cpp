@@ -289,8 +289,8 @@ int bcrt_server_create(bcrt_server** out, size_t hbm_budget_bytes, uint32_t max_
                "allocate pinned outputs");
         hcheck(hipStreamCreateWithFlags(&s->copy_in, hipStreamNonBlocking), "create stream");
         for (auto& c : s->compute) hcheck(hipStreamCreateWithFlags(&c, hipStreamNonBlocking), "create stream");
-        s->pack = std::make_unique<bcrt::CopyPool>(bcrt::pool_workers());
-        s->unpack = std::make_unique<bcrt::CopyPool>(bcrt::pool_workers());
+        s->pack = std::make_unique<bcrt::CopyPool>(bcrt::server_pool_workers());
+        s->unpack = std::make_unique<bcrt::CopyPool>(bcrt::server_pool_workers());
         s->sched = std::make_unique<bcrt::WaveScheduler>(s->cap, max_wave, bcrt::kMaxGroups, kInflight);
     } catch (const std::exception& x) {
         if (err && errlen) std::snprintf(err, errlen, "%s", x.what());
