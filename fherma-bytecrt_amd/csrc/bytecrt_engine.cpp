@@ -233,8 +233,8 @@ int bcrt_create(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, u
         t.invp = upload(*e, e->plan.invp);
         t.q = upload(*e, e->plan.q);
         // The host copies of the big tables are no longer needed.
-        std::vector<double>().swap(e->plan.zf);
-        std::vector<double>().swap(e->plan.zi);
+        std::vector<uint32_t>().swap(e->plan.zf);
+        std::vector<uint32_t>().swap(e->plan.zi);
         void* p = nullptr;
         check(hipMalloc(&p, size_t(max_batch) * 2 * g.P * g.N * sizeof(uint32_t)), "allocate residue planes");
         e->allocations.push_back(p);

@@ -94,7 +94,7 @@ __device__ __forceinline__ void gs_bfly(double& x, double& y, double w, double p
 // a length-M transform; the sub-block's outer index is o0 + sb; the twiddle
 // of group g is z[(mult << (e0 + l)) + ((o0 + sb) << l) + g].
 template <int K>
-__device__ __forceinline__ void ct_round(double (&v)[16], const double* __restrict__ z, uint32_t mult,
+__device__ __forceinline__ void ct_round(double (&v)[16], const uint32_t* __restrict__ z, uint32_t mult,
                                          uint32_t e0, uint32_t o0, double p, double pinv) {
 #pragma unroll
     for (int l = 0; l < K; ++l) {
@@ -103,7 +103,7 @@ __device__ __forceinline__ void ct_round(double (&v)[16], const double* __restri
         for (int sb = 0; sb < (16 >> K); ++sb) {
 #pragma unroll
             for (int g = 0; g < (1 << l); ++g) {
-                const double w = z[(mult << (e0 + l)) + ((o0 + sb) << l) + g];
+                const double w = double(z[(mult << (e0 + l)) + ((o0 + sb) << l) + g]);
 #pragma unroll
                 for (int x = 0; x < half; ++x) {
                     const int i0 = (sb << K) + g * 2 * half + x;
@@ -115,7 +115,7 @@ __device__ __forceinline__ void ct_round(double (&v)[16], const double* __restri
 }
 
 template <int K>
-__device__ __forceinline__ void gs_round(double (&v)[16], const double* __restrict__ z, uint32_t mult,
+__device__ __forceinline__ void gs_round(double (&v)[16], const uint32_t* __restrict__ z, uint32_t mult,
                                          uint32_t e0, uint32_t o0, double p, double pinv) {
 #pragma unroll
     for (int l = K - 1; l >= 0; --l) {
@@ -124,7 +124,7 @@ __device__ __forceinline__ void gs_round(double (&v)[16], const double* __restri
         for (int sb = 0; sb < (16 >> K); ++sb) {
 #pragma unroll
             for (int g = 0; g < (1 << l); ++g) {
-                const double w = z[(mult << (e0 + l)) + ((o0 + sb) << l) + g];
+                const double w = double(z[(mult << (e0 + l)) + ((o0 + sb) << l) + g]);
 #pragma unroll
                 for (int x = 0; x < half; ++x) {
                     const int i0 = (sb << K) + g * 2 * half + x;
@@ -369,7 +369,7 @@ __device__ __forceinline__ void fwd_cols_wg(const DevTables& t, uint32_t* __rest
     const uint32_t r0 = (bx % wg_per_prime) * CPW;
     const uint32_t cw = threadIdx.x / T, tau = threadIdx.x % T;
     uint32_t* col = plane_ptr(t, planes, prod, op, j) + size_t(r0 + cw) * N1;
-    const double* z = t.zf + size_t(j) * t.N;
+    const uint32_t* z = t.zf + size_t(j) * t.N;
     const double p = t.pd[2 * j], pinv = t.pd[2 * j + 1];
     double v[16];
 #pragma unroll
@@ -404,7 +404,7 @@ __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __rest
     const uint32_t r0 = (bx % wg_per_prime) * CPW;
     const uint32_t cw = threadIdx.x / T, tau = threadIdx.x % T;
     uint32_t* col = plane_ptr(t, planes, prod, 0, j) + size_t(r0 + cw) * N1;
-    const double* z = t.zi + size_t(j) * t.N;
+    const uint32_t* z = t.zi + size_t(j) * t.N;
     const double p = t.pd[2 * j], pinv = t.pd[2 * j + 1];
     double v[16];
     if constexpr (K2 > 0) {
@@ -435,12 +435,12 @@ __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __rest
 
 // Twiddles of a radix-16 round (K = 4, one sub-block): tw[2^l - 1 + g] is the
 // twiddle of group g at local level l, loaded ahead of the round that uses it.
-__device__ __forceinline__ void load_tw16(double (&tw)[15], const double* __restrict__ z, uint32_t mult,
+__device__ __forceinline__ void load_tw16(double (&tw)[15], const uint32_t* __restrict__ z, uint32_t mult,
                                           uint32_t e0, uint32_t o0) {
 #pragma unroll
     for (int l = 0; l < 4; ++l)
 #pragma unroll
-        for (int g = 0; g < (1 << l); ++g) tw[(1 << l) - 1 + g] = z[(mult << (e0 + l)) + (o0 << l) + g];
+        for (int g = 0; g < (1 << l); ++g) tw[(1 << l) - 1 + g] = double(z[(mult << (e0 + l)) + (o0 << l) + g]);
 }
 
 // Forward radix-16 round on two operands of the same prime and block at once:
@@ -506,8 +506,8 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
     uint32_t* pb = plane_ptr(t, planes, prod, 1, j);
     const uint32_t bl = threadIdx.x >> 4, tau = threadIdx.x & 15;
     const uint32_t mult = t.N1 + b0 + bl;
-    const double* zf = t.zf + size_t(j) * t.N;
-    const double* zi = t.zi + size_t(j) * t.N;
+    const uint32_t* zf = t.zf + size_t(j) * t.N;
+    const uint32_t* zi = t.zi + size_t(j) * t.N;
     // The exchange buffer holds both operands as 32-bit residue words (A in the first
     // half, B in the second) for the forward sub-levels, then the product as FP64.
     uint32_t* ua = reinterpret_cast<uint32_t*>(sx);

@@ -14,8 +14,8 @@ struct DevTables {
     double inv_qt1;                // 1 / qt1
     const uint32_t* primes;        // [P]
     const double* pd;              // [P][2]: p, 1/p
-    const double* zf;              // [P][N]: psi^brv(k) mod p
-    const double* zi;              // [P][N]: psi^-brv(k) mod p
+    const uint32_t* zf;            // [P][N]: psi^brv(k) mod p
+    const uint32_t* zi;            // [P][N]: psi^-brv(k) mod p
     const double* scale;           // [P]: N^-1 (P/p)^-1 mod p
     const void* enc_prime;         // [P] x 24 bytes: off (f64), 1/p rounded down (f64), p, off mod 2^32
     const int8_t* wenc;            // [PTenc][32][Kenc]

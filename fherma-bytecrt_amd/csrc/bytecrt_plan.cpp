@@ -279,12 +279,12 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
             a = mulm(a, psi, p);
             b = mulm(b, psi_inv, p);
         }
-        double* zf = pl.zf.data() + size_t(j) * N;
-        double* zi = pl.zi.data() + size_t(j) * N;
+        uint32_t* zf = pl.zf.data() + size_t(j) * N;
+        uint32_t* zi = pl.zi.data() + size_t(j) * N;
         for (uint32_t k = 0; k < N; ++k) {
             const uint32_t e = bitrev(k, g.log_n);
-            zf[k] = double(pw[e]);
-            zi[k] = double(pwi[e]);
+            zf[k] = pw[e];
+            zi[k] = pwi[e];
         }
 
         // Encode weights: r_k == 2^(8k) mod p in (-p/2, p/2], four balanced digits.
