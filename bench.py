@@ -376,9 +376,10 @@ def cpu_baseline(pkg, q, seconds: float):
 
 
 def sweep_mode(args, torch):
-    """Autotune the batch pipelining for the bench geometry (every candidate's
-    product is checked bit for bit against the first) and record the choice
-    in a pipeline table (copy it into the package to make it the default)."""
+    """Autotune the encode tile shape and then the batch pipelining for the
+    bench geometry (every candidate's product is checked bit for bit against
+    the first) and record the choices in a pipeline table (copy it into the
+    package to make it the default)."""
     sys.path.insert(0, str(ROOT))
     import __graft_entry__ as ge
     pkg = ge.load_package()
@@ -390,13 +391,16 @@ def sweep_mode(args, torch):
     c = torch.empty_like(a)
     torch.cuda.synchronize()
     stream = torch.cuda.Stream()
+    enc = tune.tune_encode_tile(eng, a, b, c, B, reps=args.reps, stream=stream)
+    for r in enc["rows"]:
+        print(json.dumps(r), flush=True)
     res = tune.tune_pipeline(eng, a, b, c, B, reps=args.reps, stream=stream)
     for r in res["rows"]:
         print(json.dumps(r), flush=True)
     out = Path(args.sweep_out)
     out.parent.mkdir(parents=True, exist_ok=True)
-    tune.record(N, W, B, res, device=torch.cuda.get_device_name(0), path=out)
-    print(json.dumps({"best": res["best"], "table": str(out)}), flush=True)
+    tune.record(N, W, B, res, device=torch.cuda.get_device_name(0), path=out, encode=enc)
+    print(json.dumps({"best": res["best"], "encode_tile": enc["best"], "table": str(out)}), flush=True)
 
 
 def main():
@@ -444,7 +448,8 @@ def main():
 
     # -- batched: one graph launch per step, pipelining from the tuned table ---
     tuned = eng.apply_tuning(B)
-    pipeline = {"chunk": eng.pipeline[0], "streams": eng.pipeline[1], "source": "table" if tuned else "default"}
+    pipeline = {"chunk": eng.pipeline[0], "streams": eng.pipeline[1], "encode_tile_ts_log": eng.encode_tile,
+                "source": "table" if tuned else "default"}
     for _ in range(max(args.warmup, 1)):
         eng.run_device(a, b, c, B, stream=stream, graph=True)
     torch.cuda.synchronize()

@@ -217,6 +217,7 @@ int bcrt_create(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, u
         t.Kcrt = g.Kcrt;
         t.Mcrt = g.Mcrt;
         t.qshift = e->plan.qshift;
+        t.enc_ts_log = std::min(5u, g.n1);
         t.qt1 = e->plan.qt1;
         t.inv_qt1 = e->plan.inv_qt1;
         t.primes = upload(*e, e->plan.primes);
@@ -402,6 +403,17 @@ extern "C" int bcrt_set_pipeline(bcrt_engine* e, uint32_t chunk, uint32_t stream
         drop_graph(e);
         e->pipe_chunk = chunk;
         e->pipe_streams = streams;
+    });
+}
+
+extern "C" int bcrt_set_encode_tile(bcrt_engine* e, uint32_t ts_log) {
+    if (!e) return BCRT_EINVAL;
+    return guarded(e, [&] {
+        const uint32_t n1 = e->tables.n1;
+        if (ts_log > 7 || ts_log > n1)
+            throw std::invalid_argument("ByteCRT encode tile needs 2^ts_log <= min(128, N/256)");
+        drop_graph(e);  // captured graphs hold the tables by value
+        e->tables.enc_ts_log = ts_log;
     });
 }
 

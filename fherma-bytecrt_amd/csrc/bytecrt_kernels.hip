@@ -160,13 +160,14 @@ size_t enc_smem_bytes(const DevTables& t) {
 
 // One encode workgroup: tile bx of operand op of product prod.
 // Encode tile bx: TR consecutive r (index mod 256) by TS consecutive s (index
-// div 256), TS = min(32, N1): each s brings TR consecutive coefficients
+// div 256), TS * TR = 128 with TS = 2^enc_ts_log <= N1 chosen per plan (32
+// unless tuned): each s brings TR consecutive coefficients
 // (TR * L contiguous input words), each (prime, r) gets TS consecutive plane
 // positions.  Tile row m is (r0 + m / TS, s0 + m % TS).
 struct EncGeom {
     uint32_t ts_log, TS, TR, r0, s0, row_words, stride_words;
     __device__ __forceinline__ EncGeom(const DevTables& t, uint32_t bx) {
-        ts_log = min(5u, t.n1);
+        ts_log = t.enc_ts_log;
         TS = 1u << ts_log;
         TR = kEncCoefs >> ts_log;
         const uint32_t rtiles = t.N2 / TR;
@@ -1050,7 +1051,7 @@ bool shapes_ok(const DevTables& t, const BatchArgs& args) {
            t.Kcrt % 32 == 0 && t.Mcrt % 32 == 0 && t.P >= 1 && t.P <= kMaxPrimesCrt && t.L >= 3 &&
            t.L <= 64 && t.Kcrt >= 4 * (t.P + 1) && t.Mcrt >= 4 * t.L + 1 && t.Kenc >= 4 * t.L &&
            t.Kcrt <= 32 * 17 && t.Mcrt <= 32 * 4 * crt_tiles_per_wave(t.Kcrt / 32, t.Mcrt / 32) &&
-           t.qshift + 32 == t.W;
+           t.qshift + 32 == t.W && t.enc_ts_log <= t.n1 && t.enc_ts_log <= 7;
 }
 
 template <int NLOG1>

@@ -10,6 +10,7 @@ namespace bcrt {
 // Device-resident constant tables of one plan, passed by value to every kernel.
 struct DevTables {
     uint32_t N, W, L, P, N1, n1, N2, Kenc, PTenc, Kcrt, Mcrt, qshift;
+    uint32_t enc_ts_log;           // encode tile: 2^enc_ts_log consecutive s by 128 >> enc_ts_log r
     uint64_t qt1;                  // (q >> qshift) + 1
     double inv_qt1;                // 1 / qt1
     const uint32_t* primes;        // [P]

@@ -1,4 +1,4 @@
-"""Pipeline autotuner for the device-resident batch path, cached per geometry.
+"""Autotuner for the device-resident batch path, cached per geometry.
 
 A batch above ``chunk`` products is split into chunks that run on ``streams``
 worker streams (``Engine.set_pipeline``), so the stages of different chunks
@@ -10,6 +10,12 @@ any capture), checks each candidate's result bit for bit against the first,
 and leaves the engine on the fastest.  ``record`` / ``apply`` keep the choice
 in a JSON table keyed by (N, W, batch), so a later engine of the same
 geometry starts from the tuned setting without timing again.
+
+``tune_encode_tile`` does the same for the kernel-level knob, the encode
+workgroup's tile shape (``Engine.set_encode_tile``: 2^ts_log consecutive s by
+128 >> ts_log consecutive r of the column-major coefficient grid), which sets
+how long the runs of its input reads and residue stores are.  It times the
+encode stage alone and checks every shape's full product against the first.
 
 Timing uses HIP events on one stream (torch's), so this module needs torch;
 the library itself does not.
@@ -67,6 +73,45 @@ def tune_pipeline(engine, a, b, c, batch: int, candidates=DEFAULT_CANDIDATES, re
     return {"best": best, "rows": rows}
 
 
+def encode_tile_candidates(engine) -> tuple:
+    """Every valid encode tile ts_log for this engine's geometry."""
+    return tuple(range(0, min(7, engine.N1.bit_length() - 1) + 1))
+
+
+def tune_encode_tile(engine, a, b, c, batch: int, candidates=None, reps: int = 5, stream=None):
+    """Time the encode stage for each tile shape (ts_log) on device operands
+    a, b (product into c).  Returns {"best": {...}, "rows": [...]}; the engine
+    is left on the best shape.  Raises if any shape's product differs from the
+    first shape's."""
+    import torch
+
+    if batch < 1 or batch > engine.max_batch:
+        raise ValueError(f"ByteCRT batch must be in [1, max_batch={engine.max_batch}]")
+    stream = stream or torch.cuda.current_stream()
+    cands = encode_tile_candidates(engine) if candidates is None else tuple(candidates)
+    ref = None
+    rows = []
+    for ts in cands:
+        engine.set_encode_tile(ts)
+        engine.run_stages(31, a=a, b=b, c=c, batch=batch, stream=stream)
+        stream.synchronize()
+        if ref is None:
+            ref = c.clone()
+        elif not torch.equal(c, ref):
+            raise RuntimeError(f"ByteCRT encode tile ts_log={ts} changed the product")
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(reps):
+            engine.run_stages(1, a=a, b=b, c=c, batch=batch, stream=stream)
+        t1.record(stream)
+        t1.synchronize()
+        rows.append({"ts_log": ts, "tile": f"{128 >> ts} r x {1 << ts} s",
+                     "encode_ms": round(t0.elapsed_time(t1) / reps, 4)})
+    best = min(rows, key=lambda r: r["encode_ms"])
+    engine.set_encode_tile(best["ts_log"])
+    return {"best": best, "rows": rows}
+
+
 def load_table(path: Path | None = None) -> dict:
     path = Path(path or os.environ.get("BCRT_PIPELINE_TABLE", TABLE))
     if not path.exists():
@@ -74,13 +119,21 @@ def load_table(path: Path | None = None) -> dict:
     return json.loads(path.read_text())
 
 
-def record(N: int, W: int, batch: int, result: dict, device: str, path: Path | None = None) -> dict:
-    """Store a tune_pipeline result in the table (returns the updated table)."""
+def record(N: int, W: int, batch: int, result: dict | None, device: str, path: Path | None = None,
+           encode: dict | None = None) -> dict:
+    """Store a tune_pipeline result and/or a tune_encode_tile result in the
+    table entry for (N, W, batch), keeping what the entry already holds
+    (returns the updated table)."""
     path = Path(path or os.environ.get("BCRT_PIPELINE_TABLE", TABLE))
     table = load_table(path)
-    best = result["best"]
-    table[_key(N, W, batch)] = {"chunk": best["chunk"], "streams": best["streams"], "ms": best["ms"],
-                                "device": device, "date": time.strftime("%Y-%m-%d")}
+    entry = dict(table.get(_key(N, W, batch), {}))
+    if result is not None:
+        best = result["best"]
+        entry.update({"chunk": best["chunk"], "streams": best["streams"], "ms": best["ms"]})
+    if encode is not None:
+        entry.update({"enc_ts_log": encode["best"]["ts_log"], "encode_ms": encode["best"]["encode_ms"]})
+    entry.update({"device": device, "date": time.strftime("%Y-%m-%d")})
+    table[_key(N, W, batch)] = entry
     path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
     return table
 
@@ -89,14 +142,24 @@ def lookup(N: int, W: int, batch: int, table: dict | None = None):
     """The tuned (chunk, streams) for this geometry and batch, or None."""
     table = load_table() if table is None else table
     e = table.get(_key(N, W, batch))
-    return None if e is None else (int(e["chunk"]), int(e["streams"]))
+    return None if e is None or "chunk" not in e else (int(e["chunk"]), int(e["streams"]))
+
+
+def lookup_encode_tile(N: int, W: int, batch: int, table: dict | None = None):
+    """The tuned encode tile ts_log for this geometry and batch, or None."""
+    table = load_table() if table is None else table
+    e = table.get(_key(N, W, batch))
+    return None if e is None or "enc_ts_log" not in e else int(e["enc_ts_log"])
 
 
 def apply(engine, batch: int, table: dict | None = None) -> bool:
-    """Put the engine on its tuned setting for ``batch`` products if the table
-    has one; returns whether it did."""
+    """Put the engine on its tuned settings for ``batch`` products (pipelining
+    and encode tile, whichever the table holds); returns whether it had any."""
+    table = load_table() if table is None else table
     hit = lookup(engine.N, engine.W, batch, table)
-    if hit is None:
-        return False
-    engine.set_pipeline(*hit)
-    return True
+    ts = lookup_encode_tile(engine.N, engine.W, batch, table)
+    if hit is not None:
+        engine.set_pipeline(*hit)
+    if ts is not None:
+        engine.set_encode_tile(ts)
+    return hit is not None or ts is not None

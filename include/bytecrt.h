@@ -82,6 +82,14 @@ int bcrt_run_device_graph(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d
  * not depend on the setting. */
 int bcrt_set_pipeline(bcrt_engine* e, uint32_t chunk, uint32_t streams);
 
+/* Encode tile shape (the autotuner's kernel-level knob; no reference
+ * counterpart: encode_mma32 fixes its tile at compile time,
+ * rns-encode-mma.cuh:148-252).  Each encode workgroup takes 128 coefficients
+ * as 2^ts_log consecutive s (index div 256) by 128 >> ts_log consecutive r
+ * (index mod 256); ts_log in [0, min(7, log2(N/256))], default min(5, ...).
+ * Results do not depend on it.  Applies to later runs and later mixes. */
+int bcrt_set_encode_tile(bcrt_engine* e, uint32_t ts_log);
+
 /* ---- Mixed batches (products of different N, W, q in one launch per stage) ----
  * No reference counterpart: the reference runs one fixed geometry per process
  * (native-batched/solve.cu:163-164).  Each group names an engine (its plan

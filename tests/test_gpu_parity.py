@@ -238,6 +238,61 @@ def test_tune_pipeline_exact(pkg, gu, tmp_path):
     assert torch.equal(out2, plain)
 
 
+@pytest.mark.parametrize("N,W", [(4096, 100), (65536, 1747)])
+def test_encode_tiles_against_oracle(pkg, orc, gu, N, W):
+    """Every encode tile shape the tuner may pick (2^ts_log s by 128 >> ts_log
+    r) writes the oracle's residue planes."""
+    q = pkg.datagen.modulus(W, N + 3)
+    e = pkg.Engine(N, W, q)
+    a = pkg.datagen.operands(N, q, 31, edges_first=True)
+    b = pkg.datagen.operands(N, q, 32, edge_fraction=0.1)
+    pl = orc.Plan(N, W, q)
+    want = [gu.to_planes(pl.encode(x), e.N1) for x in (a, b)]
+    da, db = gu.to_dev(a), gu.to_dev(b)
+    tiles = tuple(range(0, min(7, e.N1.bit_length() - 1) + 1))
+    assert len(tiles) == (5 if N == 4096 else 8)
+    for ts in tiles:
+        e.set_encode_tile(ts)
+        planes = gu.empty_dev(e.planes_words(1))
+        e.run_stages(pkg.bytecrt.STAGE_ENCODE, a=da, b=db, planes=planes)
+        got = gu.from_dev(planes).reshape(2, e.P, N)
+        for op in (0, 1):
+            np.testing.assert_array_equal(got[op], want[op], err_msg=f"ts_log {ts} operand {op}")
+    with pytest.raises(ValueError):
+        e.set_encode_tile(tiles[-1] + 1)
+
+
+def test_tune_encode_tile_exact(pkg, gu, tmp_path):
+    """The encode-tile tuner checks every shape's product, leaves the engine on
+    the fastest, and a fresh engine applying the recorded table computes the
+    same products."""
+    import importlib
+    import torch
+    tune = importlib.import_module("bytecrt_amd.tune")
+    N, W, B = 32768, 868, 4
+    q = pkg.datagen.modulus(W, 78)
+    a = np.stack([pkg.datagen.operands(N, q, 900 + i) for i in range(B)])
+    b = np.stack([pkg.datagen.operands(N, q, 950 + i) for i in range(B)])
+    e = pkg.Engine(N, W, q, max_batch=B)
+    da, db = gu.to_dev(a).view(B, N, -1), gu.to_dev(b).view(B, N, -1)
+    out = torch.empty_like(da)
+    e.run_device(da, db, out, B)
+    gu.sync()
+    plain = out.clone()
+    res = tune.tune_encode_tile(e, da, db, out, B, reps=1)
+    assert [r["ts_log"] for r in res["rows"]] == list(range(8))
+    assert e.encode_tile == res["best"]["ts_log"]
+    assert torch.equal(out, plain)
+    path = tmp_path / "table.json"
+    tune.record(N, W, B, None, device="test", path=path, encode=res)
+    f = pkg.Engine(N, W, q, max_batch=B)
+    assert f.apply_tuning(B, tune.load_table(path)) and f.encode_tile == e.encode_tile
+    out2 = torch.zeros_like(out)
+    f.run_device(da, db, out2, B, graph=True)
+    gu.sync()
+    assert torch.equal(out2, plain)
+
+
 def test_whole_call_multi_chunk(pkg, gu):
     """The host whole call at N=65,536 with B=5: three chunks through the
     staging ring, several pieces per copy, equal to the device-resident run."""

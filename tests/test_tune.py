@@ -1,5 +1,6 @@
-"""Pipeline autotuner table: record, look up and apply (CPU; the GPU timing
-itself is covered by test_gpu_parity.test_tune_pipeline_exact)."""
+"""Autotuner table: record, look up and apply (CPU; the GPU timing itself is
+covered by test_gpu_parity.test_tune_pipeline_exact and
+test_gpu_parity.test_encode_tiles_exact)."""
 import importlib.util
 import json
 from pathlib import Path
@@ -25,6 +26,9 @@ class FakeEngine:
     def set_pipeline(self, chunk, streams):
         self.calls.append((chunk, streams))
 
+    def set_encode_tile(self, ts_log):
+        self.calls.append(("tile", ts_log))
+
 
 def test_record_lookup_apply(tune, tmp_path):
     path = tmp_path / "table.json"
@@ -43,6 +47,33 @@ def test_record_lookup_apply(tune, tmp_path):
     assert set(table) == {"N65536_W1747_B128", "N32768_W868_B12"}
 
 
+def test_encode_tile_entries(tune, tmp_path):
+    path = tmp_path / "table.json"
+    enc = {"best": {"ts_log": 4, "tile": "8 r x 16 s", "encode_ms": 4.0}, "rows": []}
+    # an encode-only entry, then the pipeline added to it: both kept
+    tune.record(65536, 1747, 128, None, device="test", path=path, encode=enc)
+    table = tune.load_table(path)
+    assert tune.lookup(65536, 1747, 128, table) is None
+    assert tune.lookup_encode_tile(65536, 1747, 128, table) == 4
+    e = FakeEngine(65536, 1747)
+    assert tune.apply(e, 128, table) and e.calls == [("tile", 4)]
+    tune.record(65536, 1747, 128, {"best": {"chunk": 32, "streams": 2, "ms": 17.0}, "rows": []}, "test", path)
+    table = tune.load_table(path)
+    assert tune.lookup(65536, 1747, 128, table) == (32, 2)
+    assert tune.lookup_encode_tile(65536, 1747, 128, table) == 4
+    e = FakeEngine(65536, 1747)
+    assert tune.apply(e, 128, table) and e.calls == [(32, 2), ("tile", 4)]
+
+
+def test_encode_tile_candidates(tune):
+    class G:
+        def __init__(self, N1):
+            self.N1 = N1
+    assert tune.encode_tile_candidates(G(256)) == tuple(range(8))   # N = 65536: 128 r x 1 s .. 1 r x 128 s
+    assert tune.encode_tile_candidates(G(32)) == tuple(range(6))    # N = 8192: at most 32 s
+    assert tune.encode_tile_candidates(G(16)) == tuple(range(5))
+
+
 def test_table_from_environment(tune, tmp_path, monkeypatch):
     path = tmp_path / "env.json"
     path.write_text(json.dumps({"N8192_W220_B4": {"chunk": 2, "streams": 2, "ms": 0.1}}))
@@ -54,4 +85,7 @@ def test_shipped_table_is_valid(tune):
     table = tune.load_table(tune.TABLE)
     for key, e in table.items():
         assert key.startswith("N") and "_W" in key and "_B" in key
-        assert 1 <= e["chunk"] and 1 <= e["streams"] <= 16
+        if "chunk" in e:
+            assert 1 <= e["chunk"] and 1 <= e["streams"] <= 16
+        if "enc_ts_log" in e:
+            assert 0 <= e["enc_ts_log"] <= 7
