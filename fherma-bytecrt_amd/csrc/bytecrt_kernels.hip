@@ -180,43 +180,67 @@ struct EncGeom {
 
 constexpr uint32_t kEncSteps = kEncCoefs * 64 / kThreads;  // input words per thread (L <= 64)
 
-// Issues every input load of the tile into registers.  Word v of the tile:
-// group gi = v / G (one s), offset o = v % G within the group's TR * L
-// contiguous words; division by the magic ceil(2^32 / d) is exact for
-// v * (magic * d - 2^32) < 2^32 (v < 2^13).  Loads are unconditional (past the
-// end a thread re-reads the tile's last word) so all are in flight at once.
+// Word v of the tile (v = threadIdx.x + kThreads * it, in load order) is limb
+// v % L of the tile's coefficient c = v / L: group gi = c >> (7 - ts_log) (one
+// s), row ro = c & (TR - 1) within it.  The groups' TR * L words are contiguous
+// and N2 * L apart, so word v sits at source offset v + gi (N2 - TR) L and in
+// LDS row (ro << ts_log) + gi.  (c, limb) advance by (kThreads / L,
+// kThreads % L) per step with at most one carry, so no word needs a division.
+struct EncWalk {
+    uint32_t c, limb, dq, dr;
+    __device__ __forceinline__ explicit EncWalk(uint32_t L) {
+        c = threadIdx.x / L;
+        limb = threadIdx.x - c * L;
+        dq = kThreads / L;
+        dr = kThreads - dq * L;
+    }
+    __device__ __forceinline__ void step(uint32_t L) {
+        c += dq;
+        limb += dr;
+        const bool wrap = limb >= L;
+        limb = wrap ? limb - L : limb;
+        c += wrap ? 1u : 0u;
+    }
+};
+
+// Issues every input load of the tile into registers.  Loads are
+// unconditional (past the end a thread re-reads the tile's last word, whose
+// coefficient 127 is in the last group) so all are in flight at once.
+// Offsets stay 32-bit: N * L < 2^22; gi < 128 and (N2 - TR) L < 2^14.
 __device__ __forceinline__ void enc_load(const DevTables& t, const uint32_t* __restrict__ in, const EncGeom& e,
                                          uint32_t (&regs)[kEncSteps]) {
-    const uint32_t G = e.TR * t.L, total = kEncCoefs * t.L;
-    const uint32_t mg = 0xFFFFFFFFu / G + 1;
-    const uint32_t* src = in + (size_t(e.r0) + size_t(t.N2) * e.s0) * t.L;
-    const uint32_t group_stride = t.N2 * t.L;  // 32-bit offsets: N * L < 2^22
+    const uint32_t L = t.L, total = kEncCoefs * L, tr_log = 7 - e.ts_log;
+    const uint32_t D = (t.N2 - e.TR) * L;
+    const uint32_t* src = in + (size_t(e.r0) + size_t(t.N2) * e.s0) * L;
+    EncWalk w(L);
 #pragma unroll
     for (uint32_t it = 0; it < kEncSteps; ++it) {
         const uint32_t v = min(threadIdx.x + kThreads * it, total - 1);
-        const uint32_t gi = __umulhi(v, mg);
-        regs[it] = src[gi * group_stride + (v - gi * G)];
+        const uint32_t gi = min(w.c, kEncCoefs - 1) >> tr_log;
+        regs[it] = src[v + __umul24(gi, D)];
+        w.step(L);
     }
 }
 
 // Writes the loaded words into the LDS tile as biased bytes (x ^ 0x80 =
-// x - 128, for the signed MFMA); padding limbs L .. Kenc/4 - 1 hold biased zeros.
+// x - 128, for the signed MFMA); padding limbs L .. Kenc/4 - 1 hold biased
+// zeros.  Words past the tile go to row 0's first padding word (column
+// Kenc/4), which the GEMM never reads, so every store is unconditional.
 __device__ __forceinline__ void enc_store(const DevTables& t, const EncGeom& e, const uint32_t (&regs)[kEncSteps],
                                           uint32_t* enc_tile) {
-    const uint32_t G = e.TR * t.L, total = kEncCoefs * t.L;
-    const uint32_t mg = 0xFFFFFFFFu / G + 1, ml = 0xFFFFFFFFu / t.L + 1;
+    const uint32_t L = t.L, total = kEncCoefs * L, tr_log = 7 - e.ts_log, tr_mask = e.TR - 1;
+    EncWalk w(L);
 #pragma unroll
     for (uint32_t it = 0; it < kEncSteps; ++it) {
         const uint32_t v = threadIdx.x + kThreads * it;
-        if (v < total) {
-            const uint32_t gi = __umulhi(v, mg), o = v - gi * G;
-            const uint32_t ro = __umulhi(o, ml), limb = o - ro * t.L;
-            enc_tile[((ro << e.ts_log) + gi) * e.stride_words + limb] = regs[it] ^ 0x80808080u;
-        }
+        const uint32_t m = ((w.c & tr_mask) << e.ts_log) + (w.c >> tr_log);
+        const uint32_t idx = v < total ? __umul24(m, e.stride_words) + w.limb : e.row_words;
+        enc_tile[idx] = regs[it] ^ 0x80808080u;
+        w.step(L);
     }
-    const uint32_t pad = e.row_words - t.L;
-    for (uint32_t i = threadIdx.x; i < kEncCoefs * pad; i += kThreads)
-        enc_tile[(i / pad) * e.stride_words + t.L + i % pad] = 0x80808080u;
+    const uint32_t pad = e.row_words - L, row = threadIdx.x & (kEncCoefs - 1);
+    for (uint32_t k = threadIdx.x / kEncCoefs; k < pad; k += kThreads / kEncCoefs)
+        enc_tile[__umul24(row, e.stride_words) + L + k] = 0x80808080u;
 }
 
 // The per-prime reduction constants {off, 1/p rounded down, p, off mod 2^32}
