@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
-    ap.add_argument("--only-stage", choices=["encode", "fwd_cols", "middle", "inv_cols", "crt", "product"],
+    ap.add_argument("--only-stage", choices=["encode", "fwd_cols", "encode_fwd", "middle", "inv_cols", "crt", "product"],
                     help="profiling mode: run one stage (or the whole product) --reps times on the batch")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--sweep", action="store_true",
@@ -51,7 +51,7 @@ def parse():
     return ap.parse_args()
 
 
-STAGE_MASK = {"encode": 1, "fwd_cols": 2, "middle": 4, "inv_cols": 8, "crt": 16, "product": 31}
+STAGE_MASK = {"encode": 1, "fwd_cols": 2, "encode_fwd": 3, "middle": 4, "inv_cols": 8, "crt": 16, "product": 31}
 
 
 def profile_mode(args, torch):
@@ -125,6 +125,7 @@ def stage_bytes(P: int, L: int):
     return {
         "encode": 2 * io + 2 * R,
         "fwd_cols": 4 * R,
+        "encode_fwd": 2 * io + 2 * R,
         "middle": 3 * R,
         "inv_cols": 2 * R,
         "crt": R + io,
@@ -133,7 +134,8 @@ def stage_bytes(P: int, L: int):
 
 def stage_ops(P: int, L: int, Kenc: int, Kcrt: int, Mcrt: int):
     """Algorithmic int8 MFMA ops (2 per MAC) per product for the GEMM kernels."""
-    return {"encode": 2 * 2 * N * (4 * L) * (4 * P), "crt": 2 * N * (4 * (P + 1)) * (4 * L + 1)}
+    enc = 2 * 2 * N * (4 * L) * (4 * P)
+    return {"encode": enc, "encode_fwd": enc, "crt": 2 * N * (4 * (P + 1)) * (4 * L + 1)}
 
 
 def whole_call(eng, a, b, c_dev, torch):
@@ -469,12 +471,15 @@ def main():
     value = B / (ms_per_step / 1e3)
 
     # -- per-kernel durations on the same stream (hip events) ----------------
-    names = ["encode", "fwd_cols", "middle", "inv_cols", "crt"]
-    masks = [1, 2, 4, 8, 16]
+    # The product path's kernels (encode and forward columns fused), then the
+    # separate encode and forward-column kernels for comparison.
+    names = ["encode_fwd", "middle", "inv_cols", "crt"]
+    masks = [3, 4, 8, 16]
+    unfused_names = ["encode", "fwd_cols"]
     reps = 3
-    dur = {n: 0.0 for n in names}
+    dur = {n: 0.0 for n in names + unfused_names}
     for _ in range(reps):
-        for n, m in zip(names, masks):
+        for n, m in zip(names + unfused_names, masks + [1, 2]):
             s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             s.record(stream)
             eng.run_stages(m, a=a, b=b, c=c, batch=B, stream=stream)
@@ -484,13 +489,14 @@ def main():
     sb = stage_bytes(P, L)
     so = stage_ops(P, L, eng.Kenc, eng.Kcrt, eng.Mcrt)
     stages = {}
-    for n in names:
+    for n in names + unfused_names:
         gbs = sb[n] * B / (dur[n] * 1e-3) / 1e9
         st = {"ms": round(dur[n], 4), "hbm_GBps": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
         if n in so:
             tops = so[n] * B / (dur[n] * 1e-3) / 1e12
             st.update(int8_TOPs=round(tops, 1), mfma_frac=round(tops / INT8_PEAK_TOPS, 4))
         stages[n] = st
+    unfused = {n: stages.pop(n) for n in unfused_names}
     dom = max(names, key=lambda n: dur[n])
     if dom in so and stages[dom]["mfma_frac"] > stages[dom]["hbm_frac"]:
         roof = {"bound": "mfma", "achieved": stages[dom]["int8_TOPs"], "peak": INT8_PEAK_TOPS,
@@ -499,8 +505,10 @@ def main():
         roof = {"bound": "hbm", "achieved": stages[dom]["hbm_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": stages[dom]["hbm_frac"], "traffic": measured_traffic(dom), "kernel": dom,
                 "algorithmic_bytes_per_launch": sb[dom] * B}
-    ntt_ms = dur["fwd_cols"] + dur["middle"] + dur["inv_cols"]
-    ntt_bytes = (sb["fwd_cols"] + sb["middle"] + sb["inv_cols"]) * B
+    # NTT-only kernels of the product path (the forward column levels run
+    # inside the fused encode kernel)
+    ntt_ms = dur["middle"] + dur["inv_cols"]
+    ntt_bytes = (sb["middle"] + sb["inv_cols"]) * B
     ntt_gbs = ntt_bytes / (ntt_ms * 1e-3) / 1e9
 
     whole = None if args.no_whole_call else whole_call(eng, a, b, c, torch)
@@ -533,6 +541,7 @@ def main():
         "pipeline": pipeline,
         "exact_spot_check": exact,
         "stages": stages,
+        "unfused_stages": unfused,
         "ntt_hbm_frac": round(ntt_gbs / HBM_PEAK_GBS, 4),
         "roofline": roof,
         "whole_call": whole,

@@ -351,8 +351,12 @@ int bcrt_run_stages(bcrt_engine* e, uint32_t stage_mask, const uint32_t* d_a, co
         if ((stage_mask & 16) && !d_c) throw std::invalid_argument("ByteCRT CRT needs c");
         const bcrt::BatchArgs args{d_a, d_b, d_c, d_planes ? d_planes : e->planes, batch};
         hipStream_t s = pick(e, stream);
-        if (stage_mask & 1) check(bcrt::launch_encode(e->tables, args, s), "encode");
-        if (stage_mask & 2) check(bcrt::launch_forward_columns(e->tables, args, s), "forward columns");
+        if ((stage_mask & 3) == 3) {
+            check(bcrt::launch_encode_forward(e->tables, args, s), "encode and forward columns");
+        } else {
+            if (stage_mask & 1) check(bcrt::launch_encode(e->tables, args, s), "encode");
+            if (stage_mask & 2) check(bcrt::launch_forward_columns(e->tables, args, s), "forward columns");
+        }
         if (stage_mask & 4) check(bcrt::launch_middle(e->tables, args, s), "middle");
         if (stage_mask & 8) check(bcrt::launch_inverse_columns(e->tables, args, s), "inverse columns");
         if (stage_mask & 16) check(bcrt::launch_crt(e->tables, args, s), "crt");

@@ -166,6 +166,15 @@ size_t enc_smem_bytes(const DevTables& t) {
 // positions.  Tile row m is (r0 + m / TS, s0 + m % TS).
 struct EncGeom {
     uint32_t ts_log, TS, TR, r0, s0, row_words, stride_words;
+    __device__ __forceinline__ EncGeom(const DevTables& t, uint32_t ts, uint32_t r, uint32_t s) {
+        ts_log = ts;
+        TS = 1u << ts_log;
+        TR = kEncCoefs >> ts_log;
+        r0 = r;
+        s0 = s;
+        row_words = t.Kenc / 4;
+        stride_words = row_words + 4;
+    }
     __device__ __forceinline__ EncGeom(const DevTables& t, uint32_t bx) {
         ts_log = t.enc_ts_log;
         TS = 1u << ts_log;
@@ -613,6 +622,129 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Encode fused with the forward column levels (N1 = 256): one workgroup takes
+// a whole column r (coefficients r + 256 s, s < 256) of one operand.  The
+// column's input bytes are staged through LDS in two halves of 128
+// coefficients; each wave keeps the MFMA data fragments of its 64
+// coefficients in registers for the whole kernel.  Then, per pair of weight
+// tiles (16 primes), the waves' residues go into an FP64 exchange of 16
+// primes x 256 positions and each 16-thread group runs the column transform
+// of one prime exactly as k_fwd_cols does, storing balanced residues.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kEfS = exchange_stride<256>();
+
+size_t ef_smem_bytes(const DevTables& t) {
+    const size_t stage = size_t(kEncCoefs) * (t.Kenc + 16), exch = size_t(16) * kEfS * sizeof(double);
+    return std::max(stage, exch) + sizeof(EncPrime) * kMaxPrimesCrt;
+}
+
+template <int KS>
+__device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __restrict__ a,
+                                          const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
+                                          uint32_t r, uint32_t op, uint32_t prod, uint8_t* smem) {
+    const size_t stage = size_t(kEncCoefs) * (t.Kenc + 16), exch = size_t(16) * kEfS * sizeof(double);
+    EncPrime* cst = reinterpret_cast<EncPrime*>(smem + (stage > exch ? stage : exch));
+    for (uint32_t j = threadIdx.x; j < t.P; j += kThreads)
+        cst[j] = reinterpret_cast<const EncPrime*>(t.enc_prime)[j];
+    const uint32_t* in = (op ? b : a) + size_t(prod) * t.N * t.L;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t row = lane & 31, kh = lane >> 5;
+    const uint32_t stride_bytes = t.Kenc + 16;
+    // wave w holds coefficients s = (2 w + c) * 32 + row, c = 0, 1 (half w >> 1)
+    v4i B[2][KS];
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        const EncGeom e(t, 7, r, kEncCoefs * h);
+        uint32_t regs[kEncSteps];
+        enc_load(t, in, e, regs);
+        if (h) __syncthreads();  // the first half's fragments have been read
+        enc_store(t, e, regs, reinterpret_cast<uint32_t*>(smem));
+        __syncthreads();
+        if ((wave >> 1) == h) {
+            const uint32_t c0 = 2 * (wave & 1);
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int s2 = 0; s2 < KS; ++s2)
+                    B[c][s2] = *reinterpret_cast<const v4i*>(smem + ((c0 + c) * 32 + row) * stride_bytes + 16 * kh +
+                                                             32 * s2);
+        }
+    }
+    __syncthreads();  // the staging area becomes the exchange
+    double* X = reinterpret_cast<double*>(smem);
+    const uint32_t jt = threadIdx.x >> 4, tau = threadIdx.x & 15;
+    const uint32_t npairs = (t.PTenc + 1) / 2;
+    for (uint32_t grp = 0; grp < npairs; ++grp) {
+#pragma unroll
+        for (uint32_t u = 0; u < 2; ++u) {
+            const uint32_t pt = 2 * grp + u;
+            if (pt < t.PTenc) {
+                v4i A[KS];
+                const int8_t* wrow = t.wenc + (size_t(pt) * 32 + row) * t.Kenc + 16 * kh;
+#pragma unroll
+                for (int s2 = 0; s2 < KS; ++s2) A[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
+                double off[4], inv[4], pd[4];
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const EncPrime c = cst[min(pt * 8 + 2 * g + kh, t.P - 1)];
+                    off[g] = c.off;
+                    inv[g] = c.inv;
+                    pd[g] = double(c.p);
+                }
+                v16i acc[2] = {v16i{}, v16i{}};
+#pragma unroll
+                for (int s2 = 0; s2 < KS; ++s2) {
+                    acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[s2], B[0][s2], acc[0], 0, 0, 0);
+                    acc[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[s2], B[1][s2], acc[1], 0, 0, 0);
+                }
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const uint32_t sc = (2 * wave + c) * 32 + row;
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const int32_t lo = acc[c][4 * g] + acc[c][4 * g + 1] * 256;
+                        const int32_t hi = acc[c][4 * g + 2] + acc[c][4 * g + 3] * 256;
+                        const double ud = fma(double(hi), 65536.0, double(lo)) + off[g];
+                        const double qd = __builtin_trunc(ud * inv[g]);
+                        const uint32_t rr = uint32_t(fma(-qd, pd[g], ud));
+                        X[(8 * u + 2 * g + kh) * kEfS + skew(sc)] = double(min(rr, rr - uint32_t(pd[g])));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // forward column levels 0..7 of prime j (k_fwd_cols with N1 = 256)
+        const uint32_t j = 16 * grp + jt;
+        double* xc = X + jt * kEfS;
+        const bool live = j < t.P;
+        const uint32_t jc = live ? j : 0;
+        const uint32_t* z = t.zf + size_t(jc) * t.N;
+        const double p = t.pd[2 * jc], pinv = t.pd[2 * jc + 1];
+        double v[16];
+        if (live) {
+#pragma unroll
+            for (int x = 0; x < 16; ++x) v[x] = xc[skew(tau + 16 * x)];
+            ct_round<4>(v, z, 1, 0, 0, p, pinv);
+#pragma unroll
+            for (int x = 0; x < 16; ++x) xc[skew(tau + 16 * x)] = v[x];
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int x = 0; x < 16; ++x) v[x] = xc[skew(16 * tau + x)];
+            ct_round<4>(v, z, 1, 4, tau, p, pinv);
+            uint32_t* col = plane_ptr(t, planes, prod, op, j) + size_t(r) * t.N1;
+#pragma unroll
+            for (int x = 0; x < 16; x += 4)
+                *reinterpret_cast<uint4*>(col + 16 * tau + x) =
+                    make_uint4(uint32_t(balanced(v[x], p, pinv)), uint32_t(balanced(v[x + 1], p, pinv)),
+                               uint32_t(balanced(v[x + 2], p, pinv)), uint32_t(balanced(v[x + 3], p, pinv)));
+        }
+        __syncthreads();  // the exchange is rewritten by the next pair
+    }
+}
+
+// ---------------------------------------------------------------------------
 // CRT: 64 coefficients (consecutive column-major positions), four waves.
 // Terms t_j are in operand-A planes.  alpha = round(sum t_j / p_j): the plan's
 // guard P > 4 N (q-1)^2 puts the exact sum in [alpha, alpha + 1/4), and the
@@ -943,6 +1075,16 @@ __global__ __launch_bounds__(kThreads, 3) void k_encode(DevTables t, const uint3
     encode_dispatch(t, a, b, planes, x, y, z, smem_enc);
 }
 
+template <int KS>
+__global__ __launch_bounds__(kThreads, 3) void k_encode_fwd(DevTables t, const uint32_t* __restrict__ a,
+                                                         const uint32_t* __restrict__ b,
+                                                         uint32_t* __restrict__ planes) {
+    extern __shared__ __align__(16) uint8_t smem_ef[];
+    uint32_t r, op, prod;
+    tile_xyz(xcd_tile(blockIdx.x, gridDim.x), t.N2, 2, &r, &op, &prod);
+    encfwd_wg<KS>(t, a, b, planes, r, op, prod, smem_ef);
+}
+
 template <int NLOG1>
 __global__ __launch_bounds__(kThreads) void k_fwd_cols(DevTables t, uint32_t* __restrict__ planes) {
     extern __shared__ __align__(16) uint8_t smem_fc[];
@@ -1110,6 +1252,27 @@ hipError_t launch_encode(const DevTables& t, const BatchArgs& args, hipStream_t 
     return hipGetLastError();
 }
 
+hipError_t launch_encode_forward(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
+    if (!shapes_ok(t, args)) return hipErrorInvalidValue;
+    if (t.N1 != 256) {
+        const hipError_t e = launch_encode(t, args, stream);
+        return e != hipSuccess ? e : launch_forward_columns(t, args, stream);
+    }
+    const dim3 g(t.N2 * 2 * args.batch), blk(kThreads);
+    const size_t sm = ef_smem_bytes(t);
+    switch (t.Kenc / 32) {
+        case 1: hipLaunchKernelGGL(k_encode_fwd<1>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 2: hipLaunchKernelGGL(k_encode_fwd<2>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 3: hipLaunchKernelGGL(k_encode_fwd<3>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 4: hipLaunchKernelGGL(k_encode_fwd<4>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 5: hipLaunchKernelGGL(k_encode_fwd<5>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 6: hipLaunchKernelGGL(k_encode_fwd<6>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 7: hipLaunchKernelGGL(k_encode_fwd<7>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        default: hipLaunchKernelGGL(k_encode_fwd<8>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_forward_columns(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     if (!shapes_ok(t, args)) return hipErrorInvalidValue;
     return dispatch_cols(t, args, stream, true);
@@ -1136,8 +1299,7 @@ hipError_t launch_crt(const DevTables& t, const BatchArgs& args, hipStream_t str
 
 hipError_t launch_product(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     hipError_t e;
-    if ((e = launch_encode(t, args, stream)) != hipSuccess) return e;
-    if ((e = launch_forward_columns(t, args, stream)) != hipSuccess) return e;
+    if ((e = launch_encode_forward(t, args, stream)) != hipSuccess) return e;
     if ((e = launch_middle(t, args, stream)) != hipSuccess) return e;
     if ((e = launch_inverse_columns(t, args, stream)) != hipSuccess) return e;
     return launch_crt(t, args, stream);

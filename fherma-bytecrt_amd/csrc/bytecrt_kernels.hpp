@@ -71,6 +71,9 @@ hipError_t launch_product(const DevTables& t, const BatchArgs& args, hipStream_t
 // Individual stages (tests and profiling).
 hipError_t launch_encode(const DevTables& t, const BatchArgs& args, hipStream_t stream);
 hipError_t launch_forward_columns(const DevTables& t, const BatchArgs& args, hipStream_t stream);
+// Encode and forward column levels in one kernel (N1 = 256: one workgroup per
+// column of one operand; other geometries run the two kernels).
+hipError_t launch_encode_forward(const DevTables& t, const BatchArgs& args, hipStream_t stream);
 hipError_t launch_middle(const DevTables& t, const BatchArgs& args, hipStream_t stream);
 hipError_t launch_inverse_columns(const DevTables& t, const BatchArgs& args, hipStream_t stream);
 hipError_t launch_crt(const DevTables& t, const BatchArgs& args, hipStream_t stream);

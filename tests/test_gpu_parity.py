@@ -238,6 +238,28 @@ def test_tune_pipeline_exact(pkg, gu, tmp_path):
     assert torch.equal(out2, plain)
 
 
+@pytest.mark.parametrize("W", [1747, 1955, 300])
+def test_fused_encode_forward(pkg, gu, W):
+    """At N = 65,536 the product path runs encode and the forward column levels
+    as one kernel (a workgroup per column); its residue planes equal those of
+    the separate encode and forward-column kernels bit for bit (k-step counts
+    7, 8 and 2 of the encode GEMM)."""
+    N, B = 65536, 2
+    q = pkg.datagen.modulus(W, W + 11)
+    e = pkg.Engine(N, W, q, max_batch=B)
+    a = np.stack([pkg.datagen.operands(N, q, 60 + i, edges_first=(i == 0)) for i in range(B)])
+    b = np.stack([pkg.datagen.operands(N, q, 70 + i, edge_fraction=0.1) for i in range(B)])
+    a[0, -1] = (1 << 32) - 1
+    da, db = gu.to_dev(a), gu.to_dev(b)
+    sep, fused = gu.empty_dev(e.planes_words(B)), gu.empty_dev(e.planes_words(B))
+    e.run_stages(pkg.bytecrt.STAGE_ENCODE, a=da, b=db, planes=sep, batch=B)
+    e.run_stages(2, planes=sep, batch=B)
+    e.run_stages(3, a=da, b=db, planes=fused, batch=B)
+    got, want = gu.from_dev(fused), gu.from_dev(sep)
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, f"{bad.size} words differ, first {bad[:4].tolist()}"
+
+
 @pytest.mark.parametrize("N,W", [(4096, 100), (65536, 1747)])
 def test_encode_tiles_against_oracle(pkg, orc, gu, N, W):
     """Every encode tile shape the tuner may pick (2^ts_log s by 128 >> ts_log
