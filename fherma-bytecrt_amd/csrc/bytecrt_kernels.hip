@@ -675,14 +675,21 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
     const uint32_t jt = threadIdx.x >> 4, tau = threadIdx.x & 15;
     const uint32_t npairs = (t.PTenc + 1) / 2;
     for (uint32_t grp = 0; grp < npairs; ++grp) {
+        // both tiles' weight fragments up front: the second tile's loads land
+        // while the first one computes (a missing last tile reloads the one before)
+        v4i A2[2][KS];
+#pragma unroll
+        for (uint32_t u = 0; u < 2; ++u) {
+            const uint32_t pt = min(2 * grp + u, t.PTenc - 1);
+            const int8_t* wrow = t.wenc + (size_t(pt) * 32 + row) * t.Kenc + 16 * kh;
+#pragma unroll
+            for (int s2 = 0; s2 < KS; ++s2) A2[u][s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
+        }
 #pragma unroll
         for (uint32_t u = 0; u < 2; ++u) {
             const uint32_t pt = 2 * grp + u;
             if (pt < t.PTenc) {
-                v4i A[KS];
-                const int8_t* wrow = t.wenc + (size_t(pt) * 32 + row) * t.Kenc + 16 * kh;
-#pragma unroll
-                for (int s2 = 0; s2 < KS; ++s2) A[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
+                const v4i (&A)[KS] = A2[u];
                 double off[4], inv[4], pd[4];
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
