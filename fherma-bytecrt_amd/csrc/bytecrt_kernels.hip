@@ -6,6 +6,9 @@
 //   k_middle      forward levels n1..n-1 of A and B, pointwise product, inverse levels n-1..n1
 //   k_inv_cols    inverse levels n1-1..0, scale by N^-1 * (P/p)^-1 (CRT terms)
 //   k_crt         centered alpha + int8 MFMA CRT GEMM + carry + generic mod-q reduction
+// At N = 65,536 (N1 = 256) the product path runs k_encode_fwd, one kernel for
+// the first two (a workgroup per column), and the separate ones serve the
+// stage-level entry points and mixed batches.
 //
 // Residue planes use the "column-major" position f = r * N1 + s for natural
 // coefficient index r + N2 * s (r < N2 = 256, s < N1 = N / 256): the column
