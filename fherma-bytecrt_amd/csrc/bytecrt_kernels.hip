@@ -1068,6 +1068,9 @@ __device__ __forceinline__ void single_xyz(const DevTables& t, int stage, uint32
     stage_grid(t, 1, stage, &gx, &gy, &gz);
     stage_grid(t, gridDim.x / (gx * gy * gz), stage, &gx, &gy, &gz);  // the launch covers whole products
     tile_xyz(xcd_tile(blockIdx.x, gridDim.x), gx, gy, x, y, z);
+    *x = __builtin_amdgcn_readfirstlane(*x);  // uniform: keep them in scalar registers
+    *y = __builtin_amdgcn_readfirstlane(*y);
+    *z = __builtin_amdgcn_readfirstlane(*z);
 }
 
 // ---------------------------------------------------------------------------
@@ -1092,6 +1095,10 @@ __global__ __launch_bounds__(kThreads, 3) void k_encode_fwd(DevTables t, const u
     extern __shared__ __align__(16) uint8_t smem_ef[];
     uint32_t r, op, prod;
     tile_xyz(xcd_tile(blockIdx.x, gridDim.x), t.N2, 2, &r, &op, &prod);
+    // uniform over the workgroup: scalar registers, not VGPRs held all kernel
+    r = __builtin_amdgcn_readfirstlane(r);
+    op = __builtin_amdgcn_readfirstlane(op);
+    prod = __builtin_amdgcn_readfirstlane(prod);
     encfwd_wg<KS>(t, a, b, planes, r, op, prod, smem_ef);
 }
 
@@ -1118,12 +1125,16 @@ __global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __re
     middle_wg(t, planes, x, y, z, reinterpret_cast<double*>(smem_mid));
 }
 
+// One kernel per k-step count, chosen on the host: each gets its own register
+// allocation (one kernel switching over all seventeen spilled 88 VGPRs; at
+// Kcrt = 480 this one spills none).
+template <int KS>
 __global__ __launch_bounds__(kCrtThreads, 5) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
                                                      uint32_t* __restrict__ out) {
     extern __shared__ __align__(16) uint8_t smem_crt[];
     uint32_t x, y, z;
     single_xyz(t, 4, &x, &y, &z);
-    crt_dispatch(t, planes, out, x, z, smem_crt);
+    crt_tiles<KS>(t, planes, out, x, z, smem_crt);
 }
 
 // Finds this workgroup's group and its (x, y, z) in that group's grid, or
@@ -1302,8 +1313,27 @@ hipError_t launch_inverse_columns(const DevTables& t, const BatchArgs& args, hip
 
 hipError_t launch_crt(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     if (!shapes_ok(t, args)) return hipErrorInvalidValue;
-    const dim3 grid(t.N / kCrtCoefs * args.batch);
-    hipLaunchKernelGGL(k_crt, grid, dim3(kCrtThreads), crt_smem_bytes(t), stream, t, args.planes, args.out);
+    const dim3 g(t.N / kCrtCoefs * args.batch), blk(kCrtThreads);
+    const size_t sm = crt_smem_bytes(t);
+    switch (t.Kcrt / 32) {
+        case 1: hipLaunchKernelGGL(k_crt<1>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 2: hipLaunchKernelGGL(k_crt<2>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 3: hipLaunchKernelGGL(k_crt<3>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 4: hipLaunchKernelGGL(k_crt<4>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 5: hipLaunchKernelGGL(k_crt<5>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 6: hipLaunchKernelGGL(k_crt<6>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 7: hipLaunchKernelGGL(k_crt<7>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 8: hipLaunchKernelGGL(k_crt<8>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 9: hipLaunchKernelGGL(k_crt<9>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 10: hipLaunchKernelGGL(k_crt<10>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 11: hipLaunchKernelGGL(k_crt<11>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 12: hipLaunchKernelGGL(k_crt<12>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 13: hipLaunchKernelGGL(k_crt<13>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 14: hipLaunchKernelGGL(k_crt<14>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 15: hipLaunchKernelGGL(k_crt<15>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 16: hipLaunchKernelGGL(k_crt<16>, g, blk, sm, stream, t, args.planes, args.out); break;
+        default: hipLaunchKernelGGL(k_crt<17>, g, blk, sm, stream, t, args.planes, args.out); break;
+    }
     return hipGetLastError();
 }
 
@@ -1386,7 +1416,7 @@ hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipS
 
 hipError_t configure_kernels() {
     // CRT may need more than the 64 KiB default dynamic LDS.
-    for (const void* k : {reinterpret_cast<const void*>(k_crt), reinterpret_cast<const void*>(k_crt_groups)}) {
+    for (const void* k : {reinterpret_cast<const void*>(k_crt_groups)}) {
         const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
     }
