@@ -1223,13 +1223,17 @@ __global__ __launch_bounds__(kThreads) void k_inv_cols_groups(const GroupTable g
     }
 }
 
+template <int KS>
 __global__ __launch_bounds__(kCrtThreads, 5) void k_crt_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_crt_g[];
     uint32_t x, y, z;
     uint32_t k;
     if (!locate(g, 4, &k, &x, &y, &z)) return;
     const GroupDesc& d = g.g[k];
-    crt_dispatch(d.t, d.args.planes, d.args.out, x, z, smem_crt_g);
+    if constexpr (KS == 0)
+        crt_dispatch(d.t, d.args.planes, d.args.out, x, z, smem_crt_g);
+    else
+        crt_tiles<KS>(d.t, d.args.planes, d.args.out, x, z, smem_crt_g);
 }
 
 bool shapes_ok(const DevTables& t, const BatchArgs& args) {
@@ -1374,10 +1378,11 @@ hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipS
         table.g[k] = g;
     }
     // Specialised kernels when every group shares the shape they depend on.
-    bool same_k = true, same_n1 = true;
+    bool same_k = true, same_n1 = true, same_kc = true;
     for (uint32_t k = 1; k < ngroups; ++k) {
         same_k = same_k && groups[k].t.Kenc == groups[0].t.Kenc;
         same_n1 = same_n1 && groups[k].t.n1 == groups[0].t.n1;
+        same_kc = same_kc && groups[k].t.Kcrt == groups[0].t.Kcrt;
     }
     const uint32_t ks = same_k ? groups[0].t.Kenc / 32 : 0, n1 = same_n1 ? groups[0].t.n1 : 0;
     const dim3 blk(kThreads);
@@ -1410,16 +1415,36 @@ hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipS
         case 8: hipLaunchKernelGGL(k_inv_cols_groups<8>, dim3(total[3]), blk, xb, stream, table); break;
         default: hipLaunchKernelGGL(k_inv_cols_groups<0>, dim3(total[3]), blk, xb, stream, table); break;
     }
-    hipLaunchKernelGGL(k_crt_groups, dim3(total[4]), dim3(kCrtThreads), crt, stream, table);
+    const dim3 gc(total[4]), bc(kCrtThreads);
+    switch (same_kc ? groups[0].t.Kcrt / 32 : 0) {
+        case 1: hipLaunchKernelGGL(k_crt_groups<1>, gc, bc, crt, stream, table); break;
+        case 2: hipLaunchKernelGGL(k_crt_groups<2>, gc, bc, crt, stream, table); break;
+        case 3: hipLaunchKernelGGL(k_crt_groups<3>, gc, bc, crt, stream, table); break;
+        case 4: hipLaunchKernelGGL(k_crt_groups<4>, gc, bc, crt, stream, table); break;
+        case 5: hipLaunchKernelGGL(k_crt_groups<5>, gc, bc, crt, stream, table); break;
+        case 6: hipLaunchKernelGGL(k_crt_groups<6>, gc, bc, crt, stream, table); break;
+        case 7: hipLaunchKernelGGL(k_crt_groups<7>, gc, bc, crt, stream, table); break;
+        case 8: hipLaunchKernelGGL(k_crt_groups<8>, gc, bc, crt, stream, table); break;
+        case 9: hipLaunchKernelGGL(k_crt_groups<9>, gc, bc, crt, stream, table); break;
+        case 10: hipLaunchKernelGGL(k_crt_groups<10>, gc, bc, crt, stream, table); break;
+        case 11: hipLaunchKernelGGL(k_crt_groups<11>, gc, bc, crt, stream, table); break;
+        case 12: hipLaunchKernelGGL(k_crt_groups<12>, gc, bc, crt, stream, table); break;
+        case 13: hipLaunchKernelGGL(k_crt_groups<13>, gc, bc, crt, stream, table); break;
+        case 14: hipLaunchKernelGGL(k_crt_groups<14>, gc, bc, crt, stream, table); break;
+        case 15: hipLaunchKernelGGL(k_crt_groups<15>, gc, bc, crt, stream, table); break;
+        case 16: hipLaunchKernelGGL(k_crt_groups<16>, gc, bc, crt, stream, table); break;
+        case 17: hipLaunchKernelGGL(k_crt_groups<17>, gc, bc, crt, stream, table); break;
+        default: hipLaunchKernelGGL(k_crt_groups<0>, gc, bc, crt, stream, table); break;
+    }
     return hipGetLastError();
 }
 
 hipError_t configure_kernels() {
-    // CRT may need more than the 64 KiB default dynamic LDS.
-    for (const void* k : {reinterpret_cast<const void*>(k_crt_groups)}) {
-        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (e != hipSuccess) return e;
-    }
+    // Every kernel's dynamic LDS stays within the 64 KiB default (the largest,
+    // encode with Kenc = 256, takes 37.9 KB; the CRT at most 35.8 KB), so no
+    // attribute needs raising.
+    static_assert(size_t(kEncCoefs) * (256 + 16) + sizeof(EncPrime) * kMaxPrimesCrt <= 65536, "encode LDS");
+    static_assert(size_t(kCrtCoefs) * (32 * 17 + 16) <= 65536 && size_t(65) * kCrtCoefs * 8 <= 65536, "CRT LDS");
     return hipSuccess;
 }
 
