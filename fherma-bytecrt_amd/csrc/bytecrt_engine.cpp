@@ -522,7 +522,7 @@ extern "C" int bcrt_mix_create(bcrt_mix** out, const bcrt_group* groups, uint32_
             const uint32_t nchunks = std::max(1u, std::min(kMixChunks, in_class / kMixChunkMin));
             for (uint32_t c = 0; c < nchunks; ++c) {
                 std::vector<bcrt::GroupDesc> chunk;
-                uint32_t begin[5] = {0, 0, 0, 0, 0};
+                uint32_t begin[bcrt::kGroupStages] = {};
                 for (uint32_t k : cl) {
                     const bcrt_group& g = groups[k];
                     const uint32_t n = g.batch / nchunks + (c < g.batch % nchunks ? 1 : 0);
@@ -534,7 +534,7 @@ extern "C" int bcrt_mix_create(bcrt_mix** out, const bcrt_group* groups, uint32_
                     d.t = t;
                     d.args = bcrt::BatchArgs{g.d_a + first * io, g.d_b + first * io, g.d_c + first * io,
                                              g.engine->planes + first * pl, n};
-                    for (int st = 0; st < 5; ++st) {
+                    for (int st = 0; st < bcrt::kGroupStages; ++st) {
                         d.begin[st] = begin[st];
                         begin[st] += bcrt::group_workgroups(d.t, n, st);
                     }

@@ -1041,7 +1041,8 @@ __host__ __device__ inline void stage_grid(const DevTables& t, uint32_t batch, i
         case 1: *x = t.P * (t.N2 / cols_per_wg(t.n1)); *y = 2; *z = batch; break;
         case 2: *x = t.N1 / kMidBlocks; *y = batch; *z = t.P; break;
         case 3: *x = t.P * (t.N2 / cols_per_wg(t.n1)); *y = 1; *z = batch; break;
-        default: *x = t.N / kCrtCoefs; *y = 1; *z = batch; break;
+        case 4: *x = t.N / kCrtCoefs; *y = 1; *z = batch; break;
+        default: *x = t.N2; *y = 2; *z = batch; break;  // encode + forward columns, N1 = 256
     }
 }
 
@@ -1170,6 +1171,16 @@ __global__ __launch_bounds__(kThreads, 3) void k_encode_groups(const GroupTable 
         encode_dispatch(d.t, d.args.a, d.args.b, d.args.planes, x, y, z, smem_enc_g);
     else
         encode_wg<KS>(d.t, d.args.a, d.args.b, d.args.planes, x, y, z, smem_enc_g);
+}
+
+template <int KS>
+__global__ __launch_bounds__(kThreads, 3) void k_encode_fwd_groups(const GroupTable g) {
+    extern __shared__ __align__(16) uint8_t smem_ef_g[];
+    uint32_t r, op, prod, k;
+    if (!locate(g, 5, &k, &r, &op, &prod)) return;
+    const GroupDesc& d = g.g[k];
+    encfwd_wg<KS>(d.t, d.args.a, d.args.b, d.args.planes, __builtin_amdgcn_readfirstlane(r),
+                  __builtin_amdgcn_readfirstlane(op), __builtin_amdgcn_readfirstlane(prod), smem_ef_g);
 }
 
 template <int NLOG1>
@@ -1364,14 +1375,15 @@ hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipS
     if (ngroups == 0 || ngroups > kMaxGroups) return hipErrorInvalidValue;
     GroupTable table{};
     table.n = ngroups;
-    size_t enc = 0, crt = 0;
-    uint32_t total[5] = {0, 0, 0, 0, 0};
+    size_t enc = 0, crt = 0, ef = 0;
+    uint32_t total[kGroupStages] = {};
     for (uint32_t k = 0; k < ngroups; ++k) {
         const GroupDesc& g = groups[k];
         if (!shapes_ok(g.t, g.args)) return hipErrorInvalidValue;
         enc = std::max(enc, enc_smem_bytes(g.t));
         crt = std::max(crt, crt_smem_bytes(g.t));
-        for (int st = 0; st < 5; ++st) {
+        ef = std::max(ef, ef_smem_bytes(g.t));
+        for (int st = 0; st < kGroupStages; ++st) {
             if (g.begin[st] != total[st]) return hipErrorInvalidValue;
             total[st] += group_workgroups(g.t, g.args.batch, st);
         }
@@ -1386,25 +1398,39 @@ hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipS
     }
     const uint32_t ks = same_k ? groups[0].t.Kenc / 32 : 0, n1 = same_n1 ? groups[0].t.n1 : 0;
     const dim3 blk(kThreads);
-    switch (ks) {
-        case 1: hipLaunchKernelGGL(k_encode_groups<1>, dim3(total[0]), blk, enc, stream, table); break;
-        case 2: hipLaunchKernelGGL(k_encode_groups<2>, dim3(total[0]), blk, enc, stream, table); break;
-        case 3: hipLaunchKernelGGL(k_encode_groups<3>, dim3(total[0]), blk, enc, stream, table); break;
-        case 4: hipLaunchKernelGGL(k_encode_groups<4>, dim3(total[0]), blk, enc, stream, table); break;
-        case 5: hipLaunchKernelGGL(k_encode_groups<5>, dim3(total[0]), blk, enc, stream, table); break;
-        case 6: hipLaunchKernelGGL(k_encode_groups<6>, dim3(total[0]), blk, enc, stream, table); break;
-        case 7: hipLaunchKernelGGL(k_encode_groups<7>, dim3(total[0]), blk, enc, stream, table); break;
-        case 8: hipLaunchKernelGGL(k_encode_groups<8>, dim3(total[0]), blk, enc, stream, table); break;
-        default: hipLaunchKernelGGL(k_encode_groups<0>, dim3(total[0]), blk, enc, stream, table); break;
-    }
     const size_t xb = kExchangeBytes;
-    switch (n1) {
-        case 4: hipLaunchKernelGGL(k_fwd_cols_groups<4>, dim3(total[1]), blk, xb, stream, table); break;
-        case 5: hipLaunchKernelGGL(k_fwd_cols_groups<5>, dim3(total[1]), blk, xb, stream, table); break;
-        case 6: hipLaunchKernelGGL(k_fwd_cols_groups<6>, dim3(total[1]), blk, xb, stream, table); break;
-        case 7: hipLaunchKernelGGL(k_fwd_cols_groups<7>, dim3(total[1]), blk, xb, stream, table); break;
-        case 8: hipLaunchKernelGGL(k_fwd_cols_groups<8>, dim3(total[1]), blk, xb, stream, table); break;
-        default: hipLaunchKernelGGL(k_fwd_cols_groups<0>, dim3(total[1]), blk, xb, stream, table); break;
+    if (ks != 0 && n1 == 8) {
+        // every group has N1 = 256: encode and forward columns in one kernel
+        switch (ks) {
+            case 1: hipLaunchKernelGGL(k_encode_fwd_groups<1>, dim3(total[5]), blk, ef, stream, table); break;
+            case 2: hipLaunchKernelGGL(k_encode_fwd_groups<2>, dim3(total[5]), blk, ef, stream, table); break;
+            case 3: hipLaunchKernelGGL(k_encode_fwd_groups<3>, dim3(total[5]), blk, ef, stream, table); break;
+            case 4: hipLaunchKernelGGL(k_encode_fwd_groups<4>, dim3(total[5]), blk, ef, stream, table); break;
+            case 5: hipLaunchKernelGGL(k_encode_fwd_groups<5>, dim3(total[5]), blk, ef, stream, table); break;
+            case 6: hipLaunchKernelGGL(k_encode_fwd_groups<6>, dim3(total[5]), blk, ef, stream, table); break;
+            case 7: hipLaunchKernelGGL(k_encode_fwd_groups<7>, dim3(total[5]), blk, ef, stream, table); break;
+            default: hipLaunchKernelGGL(k_encode_fwd_groups<8>, dim3(total[5]), blk, ef, stream, table); break;
+        }
+    } else {
+        switch (ks) {
+            case 1: hipLaunchKernelGGL(k_encode_groups<1>, dim3(total[0]), blk, enc, stream, table); break;
+            case 2: hipLaunchKernelGGL(k_encode_groups<2>, dim3(total[0]), blk, enc, stream, table); break;
+            case 3: hipLaunchKernelGGL(k_encode_groups<3>, dim3(total[0]), blk, enc, stream, table); break;
+            case 4: hipLaunchKernelGGL(k_encode_groups<4>, dim3(total[0]), blk, enc, stream, table); break;
+            case 5: hipLaunchKernelGGL(k_encode_groups<5>, dim3(total[0]), blk, enc, stream, table); break;
+            case 6: hipLaunchKernelGGL(k_encode_groups<6>, dim3(total[0]), blk, enc, stream, table); break;
+            case 7: hipLaunchKernelGGL(k_encode_groups<7>, dim3(total[0]), blk, enc, stream, table); break;
+            case 8: hipLaunchKernelGGL(k_encode_groups<8>, dim3(total[0]), blk, enc, stream, table); break;
+            default: hipLaunchKernelGGL(k_encode_groups<0>, dim3(total[0]), blk, enc, stream, table); break;
+        }
+        switch (n1) {
+            case 4: hipLaunchKernelGGL(k_fwd_cols_groups<4>, dim3(total[1]), blk, xb, stream, table); break;
+            case 5: hipLaunchKernelGGL(k_fwd_cols_groups<5>, dim3(total[1]), blk, xb, stream, table); break;
+            case 6: hipLaunchKernelGGL(k_fwd_cols_groups<6>, dim3(total[1]), blk, xb, stream, table); break;
+            case 7: hipLaunchKernelGGL(k_fwd_cols_groups<7>, dim3(total[1]), blk, xb, stream, table); break;
+            case 8: hipLaunchKernelGGL(k_fwd_cols_groups<8>, dim3(total[1]), blk, xb, stream, table); break;
+            default: hipLaunchKernelGGL(k_fwd_cols_groups<0>, dim3(total[1]), blk, xb, stream, table); break;
+        }
     }
     hipLaunchKernelGGL(k_middle_groups, dim3(total[2]), blk, xb, stream, table);
     switch (n1) {

@@ -40,6 +40,11 @@ struct BatchArgs {
     uint32_t batch;
 };
 
+// Stages of a mixed launch: 0 encode, 1 forward columns, 2 middle, 3 inverse
+// columns, 4 CRT, and 5 encode fused with the forward columns (used instead
+// of 0 and 1 when every group has N1 = 256).
+constexpr int kGroupStages = 6;
+
 // One group of a mixed launch: a plan and a batch of products under it.
 // begin[s] is the group's first workgroup in stage s (prefix sums over the
 // groups, in order; group_workgroups gives each group's count).  The table
@@ -49,7 +54,7 @@ constexpr uint32_t kMaxGroups = 16;
 struct GroupDesc {
     DevTables t;
     BatchArgs args;
-    uint32_t begin[5];
+    uint32_t begin[kGroupStages];
 };
 struct GroupTable {
     GroupDesc g[kMaxGroups];

@@ -131,7 +131,7 @@ std::unique_ptr<Wave> bcrt_server::form_wave() {
     auto w = std::make_unique<Wave>();
     for (int k = 0; k < bcrt::kRings; ++k) w->start[k] = sw.start[k], w->end[k] = sw.end[k];
     size_t off = w->start[kArena];
-    uint32_t begin[5] = {0, 0, 0, 0, 0};
+    uint32_t begin[bcrt::kGroupStages] = {};
     for (size_t i = 0; i < sw.plans.size(); ++i) {
         const Plan* p = static_cast<const Plan*>(sw.plans[i]);
         const uint32_t n = uint32_t(sw.items[i].size());
@@ -144,7 +144,7 @@ std::unique_ptr<Wave> bcrt_server::form_wave() {
         bcrt::GroupDesc d{};
         d.t = p->t;
         d.args = bcrt::BatchArgs{da, db, dc, dp, n};
-        for (int st = 0; st < 5; ++st) {
+        for (int st = 0; st < bcrt::kGroupStages; ++st) {
             d.begin[st] = begin[st];
             begin[st] += bcrt::group_workgroups(d.t, n, st);
         }

@@ -61,6 +61,34 @@ def test_mix_uneven_batches(pkg, orc, gu):
     mix.close()
 
 
+def test_mix_fused_encode_class(pkg, gu):
+    """Config-4 shape in small: two N=65,536 groups under different moduli
+    (their class launches the fused encode + forward-column kernel) beside an
+    N=32,768 group (separate kernels); eager and graph runs equal each product
+    computed alone."""
+    spec = [(65536, 1747, 2, 501), (65536, 1747, 1, 502), (32768, 868, 2, 503)]
+    groups, host = [], []
+    for N, W, B, seed in spec:
+        q = pkg.datagen.modulus(W, seed)
+        e = pkg.Engine(N, W, q, max_batch=B)
+        a = np.stack([pkg.datagen.operands(N, q, 10 * seed + i, edge_fraction=0.1) for i in range(B)])
+        b = np.stack([pkg.datagen.operands(N, q, 10 * seed + 5 + i) for i in range(B)])
+        out = gu.empty_dev(B * N * e.L)
+        groups.append((e, gu.to_dev(a), gu.to_dev(b), out, B))
+        host.append((e, a, b, out))
+    want = [np.stack([alone(pkg, gu, e, a[i], b[i]) for i in range(a.shape[0])]) for e, a, b, _ in host]
+    mix = pkg.Mix(groups)
+    for graph in (False, True, True):
+        for *_, out in host:
+            out.zero_()
+        gu.sync()
+        mix.run(graph=graph)
+        gu.sync()
+        for (e, _a, _b, out), w in zip(host, want):
+            np.testing.assert_array_equal(gu.from_dev(out).reshape(-1, e.N, e.L), w, err_msg=f"N={e.N}")
+    mix.close()
+
+
 def test_mix_chunked_over_streams(pkg, orc, gu):
     """26 products: the mix is dealt into three chunks on worker streams (each
     group split across chunks); eager and graph replays equal each product
