@@ -6,9 +6,9 @@
 //   k_middle      forward levels n1..n-1 of A and B, pointwise product, inverse levels n-1..n1
 //   k_inv_cols    inverse levels n1-1..0, scale by N^-1 * (P/p)^-1 (CRT terms)
 //   k_crt         centered alpha + int8 MFMA CRT GEMM + carry + generic mod-q reduction
-// At N = 65,536 (N1 = 256) the product path runs k_encode_fwd, one kernel for
-// the first two (a workgroup per column), and the separate ones serve the
-// stage-level entry points and mixed batches.
+// For N >= 16,384 (N1 >= 64) the product path runs k_encode_fwd, one kernel
+// for the first two (a workgroup per 256 coefficients of whole columns); the
+// separate ones serve smaller N and batches, and the stage-level entry points.
 //
 // Residue planes use the "column-major" position f = r * N1 + s for natural
 // coefficient index r + N2 * s (r < N2 = 256, s < N1 = N / 256): the column
@@ -625,14 +625,15 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Encode fused with the forward column levels (N1 = 256): one workgroup takes
-// a whole column r (coefficients r + 256 s, s < 256) of one operand.  The
-// column's input bytes are staged through LDS in two halves of 128
-// coefficients; each wave keeps the MFMA data fragments of its 64
-// coefficients in registers for the whole kernel.  Then, per pair of weight
-// tiles (16 primes), the waves' residues go into an FP64 exchange of 16
-// primes x 256 positions and each 16-thread group runs the column transform
-// of one prime exactly as k_fwd_cols does, storing balanced residues.
+// Encode fused with the forward column levels (N1 = 64, 128 or 256): one
+// workgroup takes 256 coefficients, CPW = 256 / N1 whole columns r0 .. r0 +
+// CPW - 1 (coefficients r + 256 s, s < N1) of one operand.  Their input bytes
+// are staged through LDS in two halves of 128 coefficients; each wave keeps
+// the MFMA data fragments of its 64 coefficients in registers for the whole
+// kernel.  Then, per pair of weight tiles (16 primes), the waves' residues go
+// into an FP64 exchange of 16 primes x CPW columns x N1 positions and each
+// group of N1 / 16 threads runs the column transform of one (prime, column)
+// exactly as k_fwd_cols does, storing balanced residues.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kEfS = exchange_stride<256>();
 
@@ -641,10 +642,16 @@ size_t ef_smem_bytes(const DevTables& t) {
     return std::max(stage, exch) + sizeof(EncPrime) * kMaxPrimesCrt;
 }
 
-template <int KS>
+template <int KS, int NLOG1>
 __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __restrict__ a,
                                           const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
-                                          uint32_t r, uint32_t op, uint32_t prod, uint8_t* smem) {
+                                          uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem) {
+    static_assert(NLOG1 >= 6 && NLOG1 <= 8, "fused encode covers N1 = 64 .. 256");
+    constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = 256 / N1, S = exchange_stride<N1>();
+    constexpr int K2 = NLOG1 - 4;
+    constexpr uint32_t ts_log = NLOG1 < 7 ? NLOG1 : 7, TRh = kEncCoefs >> ts_log;  // r per staging half
+    static_assert(16 * CPW * S == 16 * kEfS, "exchange size");
+    const uint32_t r0 = bx * CPW;
     const size_t stage = size_t(kEncCoefs) * (t.Kenc + 16), exch = size_t(16) * kEfS * sizeof(double);
     EncPrime* cst = reinterpret_cast<EncPrime*>(smem + (stage > exch ? stage : exch));
     for (uint32_t j = threadIdx.x; j < t.P; j += kThreads)
@@ -657,7 +664,8 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
     v4i B[2][KS];
 #pragma unroll
     for (uint32_t h = 0; h < 2; ++h) {
-        const EncGeom e(t, 7, r, kEncCoefs * h);
+        // half h: coefficients 128 h .. 128 h + 127 of the workgroup
+        const EncGeom e = NLOG1 == 8 ? EncGeom(t, 7, r0, kEncCoefs * h) : EncGeom(t, ts_log, r0 + h * TRh, 0);
         uint32_t regs[kEncSteps];
         enc_load(t, in, e, regs);
         if (h) __syncthreads();  // the first half's fragments have been read
@@ -675,7 +683,7 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
     }
     __syncthreads();  // the staging area becomes the exchange
     double* X = reinterpret_cast<double*>(smem);
-    const uint32_t jt = threadIdx.x >> 4, tau = threadIdx.x & 15;
+    const uint32_t q = threadIdx.x / T, jt = q / CPW, cq = q % CPW, tau = threadIdx.x % T;
     const uint32_t npairs = (t.PTenc + 1) / 2;
     for (uint32_t grp = 0; grp < npairs; ++grp) {
         // both tiles' weight fragments up front: the second tile's loads land
@@ -709,7 +717,8 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
                 }
 #pragma unroll
                 for (int c = 0; c < 2; ++c) {
-                    const uint32_t sc = (2 * wave + c) * 32 + row;
+                    const uint32_t m = (2 * wave + c) * 32 + row;  // coefficient r0 + (m >> NLOG1) + 256 (m & (N1 - 1))
+                    const uint32_t xrow = (m >> NLOG1) * S + skew(m & (N1 - 1));
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
                         const int32_t lo = acc[c][4 * g] + acc[c][4 * g + 1] * 256;
@@ -717,15 +726,15 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
                         const double ud = fma(double(hi), 65536.0, double(lo)) + off[g];
                         const double qd = __builtin_trunc(ud * inv[g]);
                         const uint32_t rr = uint32_t(fma(-qd, pd[g], ud));
-                        X[(8 * u + 2 * g + kh) * kEfS + skew(sc)] = double(min(rr, rr - uint32_t(pd[g])));
+                        X[(8 * u + 2 * g + kh) * CPW * S + xrow] = double(min(rr, rr - uint32_t(pd[g])));
                     }
                 }
             }
         }
         __syncthreads();
-        // forward column levels 0..7 of prime j (k_fwd_cols with N1 = 256)
+        // forward column levels 0..n1-1 of prime j, column r0 + cq (k_fwd_cols)
         const uint32_t j = 16 * grp + jt;
-        double* xc = X + jt * kEfS;
+        double* xc = X + q * S;
         const bool live = j < t.P;
         const uint32_t jc = live ? j : 0;
         const uint32_t* z = t.zf + size_t(jc) * t.N;
@@ -733,17 +742,17 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
         double v[16];
         if (live) {
 #pragma unroll
-            for (int x = 0; x < 16; ++x) v[x] = xc[skew(tau + 16 * x)];
+            for (int x = 0; x < 16; ++x) v[x] = xc[skew(tau + T * x)];
             ct_round<4>(v, z, 1, 0, 0, p, pinv);
 #pragma unroll
-            for (int x = 0; x < 16; ++x) xc[skew(tau + 16 * x)] = v[x];
+            for (int x = 0; x < 16; ++x) xc[skew(tau + T * x)] = v[x];
         }
         __syncthreads();
         if (live) {
 #pragma unroll
             for (int x = 0; x < 16; ++x) v[x] = xc[skew(16 * tau + x)];
-            ct_round<4>(v, z, 1, 4, tau, p, pinv);
-            uint32_t* col = plane_ptr(t, planes, prod, op, j) + size_t(r) * t.N1;
+            ct_round<K2>(v, z, 1, 4, tau * (16 >> K2), p, pinv);
+            uint32_t* col = plane_ptr(t, planes, prod, op, j) + size_t(r0 + cq) * N1;
 #pragma unroll
             for (int x = 0; x < 16; x += 4)
                 *reinterpret_cast<uint4*>(col + 16 * tau + x) =
@@ -1042,7 +1051,7 @@ __host__ __device__ inline void stage_grid(const DevTables& t, uint32_t batch, i
         case 2: *x = t.N1 / kMidBlocks; *y = batch; *z = t.P; break;
         case 3: *x = t.P * (t.N2 / cols_per_wg(t.n1)); *y = 1; *z = batch; break;
         case 4: *x = t.N / kCrtCoefs; *y = 1; *z = batch; break;
-        default: *x = t.N2; *y = 2; *z = batch; break;  // encode + forward columns, N1 = 256
+        default: *x = t.N1; *y = 2; *z = batch; break;  // encode + forward columns: 256 / N1 columns per tile
     }
 }
 
@@ -1089,18 +1098,18 @@ __global__ __launch_bounds__(kThreads, 3) void k_encode(DevTables t, const uint3
     encode_dispatch(t, a, b, planes, x, y, z, smem_enc);
 }
 
-template <int KS>
+template <int KS, int NLOG1>
 __global__ __launch_bounds__(kThreads, 3) void k_encode_fwd(DevTables t, const uint32_t* __restrict__ a,
                                                          const uint32_t* __restrict__ b,
                                                          uint32_t* __restrict__ planes) {
     extern __shared__ __align__(16) uint8_t smem_ef[];
     uint32_t r, op, prod;
-    tile_xyz(xcd_tile(blockIdx.x, gridDim.x), t.N2, 2, &r, &op, &prod);
+    tile_xyz(xcd_tile(blockIdx.x, gridDim.x), t.N1, 2, &r, &op, &prod);
     // uniform over the workgroup: scalar registers, not VGPRs held all kernel
     r = __builtin_amdgcn_readfirstlane(r);
     op = __builtin_amdgcn_readfirstlane(op);
     prod = __builtin_amdgcn_readfirstlane(prod);
-    encfwd_wg<KS>(t, a, b, planes, r, op, prod, smem_ef);
+    encfwd_wg<KS, NLOG1>(t, a, b, planes, r, op, prod, smem_ef);
 }
 
 template <int NLOG1>
@@ -1173,13 +1182,13 @@ __global__ __launch_bounds__(kThreads, 3) void k_encode_groups(const GroupTable 
         encode_wg<KS>(d.t, d.args.a, d.args.b, d.args.planes, x, y, z, smem_enc_g);
 }
 
-template <int KS>
+template <int KS, int NLOG1>
 __global__ __launch_bounds__(kThreads, 3) void k_encode_fwd_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_ef_g[];
     uint32_t r, op, prod, k;
     if (!locate(g, 5, &k, &r, &op, &prod)) return;
     const GroupDesc& d = g.g[k];
-    encfwd_wg<KS>(d.t, d.args.a, d.args.b, d.args.planes, __builtin_amdgcn_readfirstlane(r),
+    encfwd_wg<KS, NLOG1>(d.t, d.args.a, d.args.b, d.args.planes, __builtin_amdgcn_readfirstlane(r),
                   __builtin_amdgcn_readfirstlane(op), __builtin_amdgcn_readfirstlane(prod), smem_ef_g);
 }
 
@@ -1288,23 +1297,36 @@ hipError_t launch_encode(const DevTables& t, const BatchArgs& args, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_encode_forward(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
-    if (!shapes_ok(t, args)) return hipErrorInvalidValue;
-    if (t.N1 != 256) {
-        const hipError_t e = launch_encode(t, args, stream);
-        return e != hipSuccess ? e : launch_forward_columns(t, args, stream);
-    }
-    const dim3 g(t.N2 * 2 * args.batch), blk(kThreads);
+template <int NLOG1>
+void launch_encfwd_n(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
+    const dim3 g(t.N1 * 2 * args.batch), blk(kThreads);
     const size_t sm = ef_smem_bytes(t);
     switch (t.Kenc / 32) {
-        case 1: hipLaunchKernelGGL(k_encode_fwd<1>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
-        case 2: hipLaunchKernelGGL(k_encode_fwd<2>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
-        case 3: hipLaunchKernelGGL(k_encode_fwd<3>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
-        case 4: hipLaunchKernelGGL(k_encode_fwd<4>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
-        case 5: hipLaunchKernelGGL(k_encode_fwd<5>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
-        case 6: hipLaunchKernelGGL(k_encode_fwd<6>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
-        case 7: hipLaunchKernelGGL(k_encode_fwd<7>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
-        default: hipLaunchKernelGGL(k_encode_fwd<8>, g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 1: hipLaunchKernelGGL((k_encode_fwd<1, NLOG1>), g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 2: hipLaunchKernelGGL((k_encode_fwd<2, NLOG1>), g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 3: hipLaunchKernelGGL((k_encode_fwd<3, NLOG1>), g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 4: hipLaunchKernelGGL((k_encode_fwd<4, NLOG1>), g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 5: hipLaunchKernelGGL((k_encode_fwd<5, NLOG1>), g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 6: hipLaunchKernelGGL((k_encode_fwd<6, NLOG1>), g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        case 7: hipLaunchKernelGGL((k_encode_fwd<7, NLOG1>), g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+        default: hipLaunchKernelGGL((k_encode_fwd<8, NLOG1>), g, blk, sm, stream, t, args.a, args.b, args.planes); break;
+    }
+}
+
+// The fused kernel has N1 * 2 * batch workgroups; below one per CU (N = 16,384
+// at B = 1) the separate kernels, with more and smaller workgroups, are faster.
+constexpr uint32_t kFusedMinTiles = 256;
+
+hipError_t launch_encode_forward(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
+    if (!shapes_ok(t, args)) return hipErrorInvalidValue;
+    switch (t.N1 * 2 * args.batch >= kFusedMinTiles ? t.n1 : 0) {
+        case 6: launch_encfwd_n<6>(t, args, stream); break;
+        case 7: launch_encfwd_n<7>(t, args, stream); break;
+        case 8: launch_encfwd_n<8>(t, args, stream); break;
+        default: {
+            const hipError_t e = launch_encode(t, args, stream);
+            return e != hipSuccess ? e : launch_forward_columns(t, args, stream);
+        }
     }
     return hipGetLastError();
 }
@@ -1371,6 +1393,21 @@ uint32_t group_workgroups(const DevTables& t, uint32_t batch, int stage) {
     return (x * y * z + 7) / 8 * 8;  // padded to whole rounds of the 8 XCDs
 }
 
+template <int NLOG1>
+void launch_encfwd_groups(uint32_t ks, uint32_t tiles, size_t sm, const GroupTable& table, hipStream_t stream) {
+    const dim3 g(tiles), blk(kThreads);
+    switch (ks) {
+        case 1: hipLaunchKernelGGL((k_encode_fwd_groups<1, NLOG1>), g, blk, sm, stream, table); break;
+        case 2: hipLaunchKernelGGL((k_encode_fwd_groups<2, NLOG1>), g, blk, sm, stream, table); break;
+        case 3: hipLaunchKernelGGL((k_encode_fwd_groups<3, NLOG1>), g, blk, sm, stream, table); break;
+        case 4: hipLaunchKernelGGL((k_encode_fwd_groups<4, NLOG1>), g, blk, sm, stream, table); break;
+        case 5: hipLaunchKernelGGL((k_encode_fwd_groups<5, NLOG1>), g, blk, sm, stream, table); break;
+        case 6: hipLaunchKernelGGL((k_encode_fwd_groups<6, NLOG1>), g, blk, sm, stream, table); break;
+        case 7: hipLaunchKernelGGL((k_encode_fwd_groups<7, NLOG1>), g, blk, sm, stream, table); break;
+        default: hipLaunchKernelGGL((k_encode_fwd_groups<8, NLOG1>), g, blk, sm, stream, table); break;
+    }
+}
+
 hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipStream_t stream) {
     if (ngroups == 0 || ngroups > kMaxGroups) return hipErrorInvalidValue;
     GroupTable table{};
@@ -1399,17 +1436,12 @@ hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipS
     const uint32_t ks = same_k ? groups[0].t.Kenc / 32 : 0, n1 = same_n1 ? groups[0].t.n1 : 0;
     const dim3 blk(kThreads);
     const size_t xb = kExchangeBytes;
-    if (ks != 0 && n1 == 8) {
-        // every group has N1 = 256: encode and forward columns in one kernel
-        switch (ks) {
-            case 1: hipLaunchKernelGGL(k_encode_fwd_groups<1>, dim3(total[5]), blk, ef, stream, table); break;
-            case 2: hipLaunchKernelGGL(k_encode_fwd_groups<2>, dim3(total[5]), blk, ef, stream, table); break;
-            case 3: hipLaunchKernelGGL(k_encode_fwd_groups<3>, dim3(total[5]), blk, ef, stream, table); break;
-            case 4: hipLaunchKernelGGL(k_encode_fwd_groups<4>, dim3(total[5]), blk, ef, stream, table); break;
-            case 5: hipLaunchKernelGGL(k_encode_fwd_groups<5>, dim3(total[5]), blk, ef, stream, table); break;
-            case 6: hipLaunchKernelGGL(k_encode_fwd_groups<6>, dim3(total[5]), blk, ef, stream, table); break;
-            case 7: hipLaunchKernelGGL(k_encode_fwd_groups<7>, dim3(total[5]), blk, ef, stream, table); break;
-            default: hipLaunchKernelGGL(k_encode_fwd_groups<8>, dim3(total[5]), blk, ef, stream, table); break;
+    if (ks != 0 && n1 >= 6 && total[5] >= kFusedMinTiles) {
+        // every group has the same Kenc and N1 >= 64: encode and forward columns in one kernel
+        switch (n1) {
+            case 6: launch_encfwd_groups<6>(ks, total[5], ef, table, stream); break;
+            case 7: launch_encfwd_groups<7>(ks, total[5], ef, table, stream); break;
+            default: launch_encfwd_groups<8>(ks, total[5], ef, table, stream); break;
         }
     } else {
         switch (ks) {

@@ -163,9 +163,10 @@ int bcrt_plan_host(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, uint32
  * ([batch][2][P][N] u32, column-major positions) live in d_planes (NULL = the
  * engine workspace).  stage_mask bits: 1 encode, 2 forward columns, 4 middle
  * (forward blocks, pointwise, inverse blocks), 8 inverse columns + CRT scaling,
- * 16 CRT.  A mask with both 1 and 2 runs them as one fused kernel when N = 65,536
- * (N/256 = 256: a workgroup per column), as the product path does; 1 or 2
- * alone runs the separate kernels, which write the same planes.
+ * 16 CRT.  A mask with both 1 and 2 runs them as one fused kernel when
+ * N >= 16,384 and (N/256) * 2 * batch >= 256 (a workgroup per 256
+ * coefficients of whole columns), as the product path does; 1 or 2 alone runs
+ * the separate kernels, which write the same planes.
  * The stages replace, in order, encode_mma32 (rns-encode-mma.cuh:148-252),
  * forward_first_coalesced (ntt-pipeline.cuh:73-94), fused_middle
  * (ntt-pipeline.cuh:96-120), inverse_weighted (ntt-weighted.cuh:10-44) and

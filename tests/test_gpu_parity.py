@@ -238,13 +238,13 @@ def test_tune_pipeline_exact(pkg, gu, tmp_path):
     assert torch.equal(out2, plain)
 
 
-@pytest.mark.parametrize("W", [1747, 1955, 300])
-def test_fused_encode_forward(pkg, gu, W):
-    """At N = 65,536 the product path runs encode and the forward column levels
-    as one kernel (a workgroup per column); its residue planes equal those of
-    the separate encode and forward-column kernels bit for bit (k-step counts
-    7, 8 and 2 of the encode GEMM)."""
-    N, B = 65536, 2
+@pytest.mark.parametrize("N,W", [(65536, 1747), (65536, 1955), (65536, 300), (32768, 868), (16384, 436)])
+def test_fused_encode_forward(pkg, gu, N, W):
+    """For N >= 16,384 the product path runs encode and the forward column
+    levels as one kernel (a workgroup per 256 coefficients of whole columns);
+    its residue planes equal those of the separate encode and forward-column
+    kernels bit for bit (N1 = 256, 128, 64; encode k-step counts 7, 8, 2, 4)."""
+    B = 2
     q = pkg.datagen.modulus(W, W + 11)
     e = pkg.Engine(N, W, q, max_batch=B)
     a = np.stack([pkg.datagen.operands(N, q, 60 + i, edges_first=(i == 0)) for i in range(B)])
