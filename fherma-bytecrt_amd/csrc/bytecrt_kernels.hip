@@ -1162,8 +1162,11 @@ __device__ __forceinline__ bool locate(const GroupTable& g, int stage, uint32_t*
     stage_grid(g.g[k].t, g.g[k].args.batch, stage, &gx, &gy, &gz);
     const uint32_t count = gx * gy * gz;
     const uint32_t tile = xcd_tile(blockIdx.x - g.g[k].begin[stage], (count + 7) / 8 * 8);
-    *k_out = k;
+    *k_out = __builtin_amdgcn_readfirstlane(k);  // uniform: scalar registers
     tile_xyz(tile, gx, gy, x, y, z);
+    *x = __builtin_amdgcn_readfirstlane(*x);
+    *y = __builtin_amdgcn_readfirstlane(*y);
+    *z = __builtin_amdgcn_readfirstlane(*z);
     return tile < count;
 }
 
