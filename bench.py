@@ -378,10 +378,11 @@ def cpu_baseline(pkg, q, seconds: float):
 
 
 def sweep_mode(args, torch):
-    """Autotune the encode tile shape and then the batch pipelining for the
-    bench geometry (every candidate's product is checked bit for bit against
-    the first) and record the choices in a pipeline table (copy it into the
-    package to make it the default)."""
+    """Autotune the encode tile shape, the fusion of encode with the forward
+    columns and then the batch pipelining for the bench geometry (every
+    candidate's product is checked bit for bit against the first) and record
+    the choices in a pipeline table (copy it into the package to make it the
+    default)."""
     sys.path.insert(0, str(ROOT))
     import __graft_entry__ as ge
     pkg = ge.load_package()
@@ -396,13 +397,17 @@ def sweep_mode(args, torch):
     enc = tune.tune_encode_tile(eng, a, b, c, B, reps=args.reps, stream=stream)
     for r in enc["rows"]:
         print(json.dumps(r), flush=True)
+    fus = tune.tune_fusion(eng, a, b, c, B, reps=args.reps, stream=stream)
+    for r in fus["rows"]:
+        print(json.dumps(r), flush=True)
     res = tune.tune_pipeline(eng, a, b, c, B, reps=args.reps, stream=stream)
     for r in res["rows"]:
         print(json.dumps(r), flush=True)
     out = Path(args.sweep_out)
     out.parent.mkdir(parents=True, exist_ok=True)
-    tune.record(N, W, B, res, device=torch.cuda.get_device_name(0), path=out, encode=enc)
-    print(json.dumps({"best": res["best"], "encode_tile": enc["best"], "table": str(out)}), flush=True)
+    tune.record(N, W, B, res, device=torch.cuda.get_device_name(0), path=out, encode=enc, fusion=fus)
+    print(json.dumps({"best": res["best"], "encode_tile": enc["best"], "fusion": fus["best"], "table": str(out)}),
+          flush=True)
 
 
 def main():
@@ -451,7 +456,7 @@ def main():
     # -- batched: one graph launch per step, pipelining from the tuned table ---
     tuned = eng.apply_tuning(B)
     pipeline = {"chunk": eng.pipeline[0], "streams": eng.pipeline[1], "encode_tile_ts_log": eng.encode_tile,
-                "source": "table" if tuned else "default"}
+                "fuse_min_tiles": eng.fuse_min_tiles, "source": "table" if tuned else "default"}
     for _ in range(max(args.warmup, 1)):
         eng.run_device(a, b, c, B, stream=stream, graph=True)
     torch.cuda.synchronize()

@@ -30,7 +30,7 @@ EXPORTED = (
     "bcrt_create", "bcrt_destroy", "bcrt_run", "bcrt_run_sink", "bcrt_run_device", "bcrt_run_device_graph",
     "bcrt_synchronize", "bcrt_last_error", "bcrt_geometry", "bcrt_plan_tables",
     "bcrt_run_stages", "bcrt_planes_words", "bcrt_mfma_probe", "bcrt_plan_host", "bcrt_set_pipeline",
-    "bcrt_set_encode_tile",
+    "bcrt_set_encode_tile", "bcrt_set_fusion",
     "bcrt_mix_create", "bcrt_mix_run", "bcrt_mix_last_error", "bcrt_mix_destroy",
     "bcrt_server_create", "bcrt_server_submit", "bcrt_server_wait", "bcrt_server_stats",
     "bcrt_server_last_error", "bcrt_server_destroy",
@@ -84,6 +84,8 @@ def load_library():
     lib.bcrt_set_pipeline.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
     lib.bcrt_set_encode_tile.restype = ctypes.c_int
     lib.bcrt_set_encode_tile.argtypes = [vp, ctypes.c_uint32]
+    lib.bcrt_set_fusion.restype = ctypes.c_int
+    lib.bcrt_set_fusion.argtypes = [vp, ctypes.c_uint32]
     lib.bcrt_plan_host.restype = ctypes.c_int
     lib.bcrt_plan_host.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p,
                                    _u32p, _u32p, _u32p, _u32p, ctypes.c_char_p, ctypes.c_size_t]
@@ -156,6 +158,7 @@ class Engine:
         self._h = handle
         self.pipeline = (16, 4)  # the library's default (chunk, streams)
         self.encode_tile = min(5, (N // 256).bit_length() - 1)  # the library's default ts_log
+        self.fuse_min_tiles = 256  # the library's default fusion threshold
         g = np.zeros(10, dtype=np.uint32)
         lib.bcrt_geometry(self._h, g.ctypes.data_as(_u32p))
         (self.N, self.W, self.L, self.P, self.N1, self.N2, self.Kenc, self.Kcrt, self.Mcrt,
@@ -222,6 +225,14 @@ class Engine:
         """Encode tile of 2^ts_log consecutive s by 128 >> ts_log consecutive r."""
         self._check(load_library().bcrt_set_encode_tile(self._h, ts_log))
         self.encode_tile = ts_log
+
+    FUSE_NEVER = 0xFFFFFFFF
+
+    def set_fusion(self, min_tiles: int):
+        """Fuse encode with the forward columns from `min_tiles` workgroups
+        ((N/256) * 2 * batch; 0 = whenever N >= 16,384, FUSE_NEVER = never)."""
+        self._check(load_library().bcrt_set_fusion(self._h, min_tiles))
+        self.fuse_min_tiles = min_tiles
 
     def apply_tuning(self, batch: int, table: dict | None = None) -> bool:
         """Use the pipelining the autotuner recorded for this geometry and batch

@@ -218,6 +218,7 @@ int bcrt_create(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, u
         t.Mcrt = g.Mcrt;
         t.qshift = e->plan.qshift;
         t.enc_ts_log = std::min(5u, g.n1);
+        t.fuse_min_tiles = bcrt::kFusedMinTiles;
         t.qt1 = e->plan.qt1;
         t.inv_qt1 = e->plan.inv_qt1;
         t.primes = upload(*e, e->plan.primes);
@@ -418,6 +419,14 @@ extern "C" int bcrt_set_encode_tile(bcrt_engine* e, uint32_t ts_log) {
             throw std::invalid_argument("ByteCRT encode tile needs 2^ts_log <= min(128, N/256)");
         drop_graph(e);  // captured graphs hold the tables by value
         e->tables.enc_ts_log = ts_log;
+    });
+}
+
+extern "C" int bcrt_set_fusion(bcrt_engine* e, uint32_t min_tiles) {
+    if (!e) return BCRT_EINVAL;
+    return guarded(e, [&] {
+        drop_graph(e);  // captured graphs hold the tables by value
+        e->tables.fuse_min_tiles = min_tiles;
     });
 }
 

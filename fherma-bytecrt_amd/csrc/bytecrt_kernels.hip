@@ -1316,13 +1316,12 @@ void launch_encfwd_n(const DevTables& t, const BatchArgs& args, hipStream_t stre
     }
 }
 
-// The fused kernel has N1 * 2 * batch workgroups; below one per CU (N = 16,384
-// at B = 1) the separate kernels, with more and smaller workgroups, are faster.
-constexpr uint32_t kFusedMinTiles = 256;
-
 hipError_t launch_encode_forward(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     if (!shapes_ok(t, args)) return hipErrorInvalidValue;
-    switch (t.N1 * 2 * args.batch >= kFusedMinTiles ? t.n1 : 0) {
+    // The fused kernel has N1 * 2 * batch workgroups; below about one per CU
+    // (N = 16,384 at B = 1) the separate kernels, with more and smaller
+    // workgroups, are faster (default threshold kFusedMinTiles; tunable).
+    switch (t.N1 * 2 * args.batch >= t.fuse_min_tiles ? t.n1 : 0) {
         case 6: launch_encfwd_n<6>(t, args, stream); break;
         case 7: launch_encfwd_n<7>(t, args, stream); break;
         case 8: launch_encfwd_n<8>(t, args, stream); break;
@@ -1439,7 +1438,7 @@ hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipS
     const uint32_t ks = same_k ? groups[0].t.Kenc / 32 : 0, n1 = same_n1 ? groups[0].t.n1 : 0;
     const dim3 blk(kThreads);
     const size_t xb = kExchangeBytes;
-    if (ks != 0 && n1 >= 6 && total[5] >= kFusedMinTiles) {
+    if (ks != 0 && n1 >= 6 && total[5] >= groups[0].t.fuse_min_tiles) {
         // every group has the same Kenc and N1 >= 64: encode and forward columns in one kernel
         switch (n1) {
             case 6: launch_encfwd_groups<6>(ks, total[5], ef, table, stream); break;

@@ -11,6 +11,7 @@ namespace bcrt {
 struct DevTables {
     uint32_t N, W, L, P, N1, n1, N2, Kenc, PTenc, Kcrt, Mcrt, qshift;
     uint32_t enc_ts_log;           // encode tile: 2^enc_ts_log consecutive s by 128 >> enc_ts_log r
+    uint32_t fuse_min_tiles;       // host: fused encode + forward columns from this many workgroups
     uint64_t qt1;                  // (q >> qshift) + 1
     double inv_qt1;                // 1 / qt1
     const uint32_t* primes;        // [P]
@@ -76,8 +77,10 @@ hipError_t launch_product(const DevTables& t, const BatchArgs& args, hipStream_t
 // Individual stages (tests and profiling).
 hipError_t launch_encode(const DevTables& t, const BatchArgs& args, hipStream_t stream);
 hipError_t launch_forward_columns(const DevTables& t, const BatchArgs& args, hipStream_t stream);
-// Encode and forward column levels in one kernel (N1 = 256: one workgroup per
-// column of one operand; other geometries run the two kernels).
+// Encode and forward column levels in one kernel (N1 >= 64: one workgroup per
+// 256 coefficients of whole columns of one operand) when that is at least
+// t.fuse_min_tiles workgroups; otherwise the two kernels.
+constexpr uint32_t kFusedMinTiles = 256;
 hipError_t launch_encode_forward(const DevTables& t, const BatchArgs& args, hipStream_t stream);
 hipError_t launch_middle(const DevTables& t, const BatchArgs& args, hipStream_t stream);
 hipError_t launch_inverse_columns(const DevTables& t, const BatchArgs& args, hipStream_t stream);

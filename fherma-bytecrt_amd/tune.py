@@ -16,6 +16,10 @@ workgroup's tile shape (``Engine.set_encode_tile``: 2^ts_log consecutive s by
 128 >> ts_log consecutive r of the column-major coefficient grid), which sets
 how long the runs of its input reads and residue stores are.  It times the
 encode stage alone and checks every shape's full product against the first.
+``tune_fusion`` picks the kernel structure of the first two stages: encode and
+the forward column levels fused into one kernel (N >= 16,384; one workgroup
+per 256 coefficients of whole columns) or run as two kernels, which wins for
+batches too small to fill the GPU with the fused kernel's workgroups.
 
 Timing uses HIP events on one stream (torch's), so this module needs torch;
 the library itself does not.
@@ -112,6 +116,42 @@ def tune_encode_tile(engine, a, b, c, batch: int, candidates=None, reps: int = 5
     return {"best": best, "rows": rows}
 
 
+FUSE_ALWAYS, FUSE_NEVER = 0, 0xFFFFFFFF
+
+
+def tune_fusion(engine, a, b, c, batch: int, reps: int = 5, stream=None):
+    """Time encode + forward columns fused and as two kernels for ``batch``
+    products of device operands a, b (product into c); checks that both give
+    the same product and leaves the engine on the faster.  Returns
+    {"best": {...}, "rows": [...]}."""
+    import torch
+
+    if batch < 1 or batch > engine.max_batch:
+        raise ValueError(f"ByteCRT batch must be in [1, max_batch={engine.max_batch}]")
+    stream = stream or torch.cuda.current_stream()
+    ref = None
+    rows = []
+    for setting, name in ((FUSE_ALWAYS, "fused"), (FUSE_NEVER, "separate")):
+        engine.set_fusion(setting)
+        engine.run_stages(31, a=a, b=b, c=c, batch=batch, stream=stream)
+        stream.synchronize()
+        if ref is None:
+            ref = c.clone()
+        elif not torch.equal(c, ref):
+            raise RuntimeError(f"ByteCRT {name} encode changed the product")
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(reps):
+            engine.run_stages(3, a=a, b=b, c=c, batch=batch, stream=stream)
+        t1.record(stream)
+        t1.synchronize()
+        rows.append({"fuse_min_tiles": setting, "structure": name,
+                     "encode_fwd_ms": round(t0.elapsed_time(t1) / reps, 4)})
+    best = min(rows, key=lambda r: r["encode_fwd_ms"])
+    engine.set_fusion(best["fuse_min_tiles"])
+    return {"best": best, "rows": rows}
+
+
 def load_table(path: Path | None = None) -> dict:
     path = Path(path or os.environ.get("BCRT_PIPELINE_TABLE", TABLE))
     if not path.exists():
@@ -120,8 +160,8 @@ def load_table(path: Path | None = None) -> dict:
 
 
 def record(N: int, W: int, batch: int, result: dict | None, device: str, path: Path | None = None,
-           encode: dict | None = None) -> dict:
-    """Store a tune_pipeline result and/or a tune_encode_tile result in the
+           encode: dict | None = None, fusion: dict | None = None) -> dict:
+    """Store tune_pipeline, tune_encode_tile and/or tune_fusion results in the
     table entry for (N, W, batch), keeping what the entry already holds
     (returns the updated table)."""
     path = Path(path or os.environ.get("BCRT_PIPELINE_TABLE", TABLE))
@@ -132,6 +172,9 @@ def record(N: int, W: int, batch: int, result: dict | None, device: str, path: P
         entry.update({"chunk": best["chunk"], "streams": best["streams"], "ms": best["ms"]})
     if encode is not None:
         entry.update({"enc_ts_log": encode["best"]["ts_log"], "encode_ms": encode["best"]["encode_ms"]})
+    if fusion is not None:
+        entry.update({"fuse_min_tiles": fusion["best"]["fuse_min_tiles"],
+                      "encode_fwd_ms": fusion["best"]["encode_fwd_ms"]})
     entry.update({"device": device, "date": time.strftime("%Y-%m-%d")})
     table[_key(N, W, batch)] = entry
     path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
@@ -152,14 +195,25 @@ def lookup_encode_tile(N: int, W: int, batch: int, table: dict | None = None):
     return None if e is None or "enc_ts_log" not in e else int(e["enc_ts_log"])
 
 
+def lookup_fusion(N: int, W: int, batch: int, table: dict | None = None):
+    """The tuned fusion threshold for this geometry and batch, or None."""
+    table = load_table() if table is None else table
+    e = table.get(_key(N, W, batch))
+    return None if e is None or "fuse_min_tiles" not in e else int(e["fuse_min_tiles"])
+
+
 def apply(engine, batch: int, table: dict | None = None) -> bool:
-    """Put the engine on its tuned settings for ``batch`` products (pipelining
-    and encode tile, whichever the table holds); returns whether it had any."""
+    """Put the engine on its tuned settings for ``batch`` products (pipelining,
+    encode tile and fusion, whichever the table holds); returns whether it
+    had any."""
     table = load_table() if table is None else table
     hit = lookup(engine.N, engine.W, batch, table)
     ts = lookup_encode_tile(engine.N, engine.W, batch, table)
+    fu = lookup_fusion(engine.N, engine.W, batch, table)
     if hit is not None:
         engine.set_pipeline(*hit)
     if ts is not None:
         engine.set_encode_tile(ts)
-    return hit is not None or ts is not None
+    if fu is not None:
+        engine.set_fusion(fu)
+    return hit is not None or ts is not None or fu is not None

@@ -90,6 +90,16 @@ int bcrt_set_pipeline(bcrt_engine* e, uint32_t chunk, uint32_t streams);
  * Results do not depend on it.  Applies to later runs and later mixes. */
 int bcrt_set_encode_tile(bcrt_engine* e, uint32_t ts_log);
 
+/* Kernel structure of the first two stages (the autotuner's other
+ * kernel-level knob; no reference counterpart, the reference always runs
+ * encode_mma32 and forward_first_coalesced as two kernels,
+ * ntt-pipeline.cuh:73-94).  For N >= 16,384 encode and the forward column
+ * levels run as one fused kernel when it has at least `min_tiles` workgroups
+ * ((N/256) * 2 * batch; default 256), otherwise as two kernels.  0 fuses
+ * whenever the geometry allows, 0xFFFFFFFF never.  Results do not depend on
+ * it.  Applies to later runs and later mixes. */
+int bcrt_set_fusion(bcrt_engine* e, uint32_t min_tiles);
+
 /* ---- Mixed batches (products of different N, W, q in one launch per stage) ----
  * No reference counterpart: the reference runs one fixed geometry per process
  * (native-batched/solve.cu:163-164).  Each group names an engine (its plan

@@ -315,6 +315,38 @@ def test_tune_encode_tile_exact(pkg, gu, tmp_path):
     assert torch.equal(out2, plain)
 
 
+@pytest.mark.parametrize("N,W,B", [(16384, 436, 1), (32768, 868, 3)])
+def test_tune_fusion_exact(pkg, gu, tmp_path, N, W, B):
+    """The fusion tuner runs encode + forward columns fused and as two kernels,
+    keeps the product bit-exact either way, leaves the engine on the faster,
+    and a fresh engine applying the recorded choice computes the same
+    products through the graph path."""
+    import importlib
+    import torch
+    tune = importlib.import_module("bytecrt_amd.tune")
+    q = pkg.datagen.modulus(W, 79)
+    a = np.stack([pkg.datagen.operands(N, q, 1200 + i, edge_fraction=0.1) for i in range(B)])
+    b = np.stack([pkg.datagen.operands(N, q, 1300 + i) for i in range(B)])
+    e = pkg.Engine(N, W, q, max_batch=B)
+    da, db = gu.to_dev(a).view(B, N, -1), gu.to_dev(b).view(B, N, -1)
+    out = torch.empty_like(da)
+    e.run_device(da, db, out, B)
+    gu.sync()
+    plain = out.clone()
+    res = tune.tune_fusion(e, da, db, out, B, reps=2)
+    assert [r["structure"] for r in res["rows"]] == ["fused", "separate"]
+    assert e.fuse_min_tiles == res["best"]["fuse_min_tiles"]
+    assert torch.equal(out, plain)
+    path = tmp_path / "table.json"
+    tune.record(N, W, B, None, device="test", path=path, fusion=res)
+    f = pkg.Engine(N, W, q, max_batch=B)
+    assert f.apply_tuning(B, tune.load_table(path)) and f.fuse_min_tiles == e.fuse_min_tiles
+    out2 = torch.zeros_like(out)
+    f.run_device(da, db, out2, B, graph=True)
+    gu.sync()
+    assert torch.equal(out2, plain)
+
+
 def test_whole_call_multi_chunk(pkg, gu):
     """The host whole call at N=65,536 with B=5: three chunks through the
     staging ring, several pieces per copy, equal to the device-resident run."""

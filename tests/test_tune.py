@@ -29,6 +29,9 @@ class FakeEngine:
     def set_encode_tile(self, ts_log):
         self.calls.append(("tile", ts_log))
 
+    def set_fusion(self, min_tiles):
+        self.calls.append(("fuse", min_tiles))
+
 
 def test_record_lookup_apply(tune, tmp_path):
     path = tmp_path / "table.json"
@@ -65,6 +68,17 @@ def test_encode_tile_entries(tune, tmp_path):
     assert tune.apply(e, 128, table) and e.calls == [(32, 2), ("tile", 4)]
 
 
+def test_fusion_entries(tune, tmp_path):
+    path = tmp_path / "table.json"
+    fus = {"best": {"fuse_min_tiles": tune.FUSE_NEVER, "structure": "separate", "encode_fwd_ms": 0.01}, "rows": []}
+    tune.record(16384, 436, 1, None, device="test", path=path, fusion=fus)
+    table = tune.load_table(path)
+    assert tune.lookup_fusion(16384, 436, 1, table) == 0xFFFFFFFF
+    assert tune.lookup(16384, 436, 1, table) is None
+    e = FakeEngine(16384, 436)
+    assert tune.apply(e, 1, table) and e.calls == [("fuse", 0xFFFFFFFF)]
+
+
 def test_encode_tile_candidates(tune):
     class G:
         def __init__(self, N1):
@@ -89,3 +103,5 @@ def test_shipped_table_is_valid(tune):
             assert 1 <= e["chunk"] and 1 <= e["streams"] <= 16
         if "enc_ts_log" in e:
             assert 0 <= e["enc_ts_log"] <= 7
+        if "fuse_min_tiles" in e:
+            assert 0 <= e["fuse_min_tiles"] <= 0xFFFFFFFF
