@@ -52,6 +52,11 @@ constexpr int kEncChains = 2;        // independent MFMA accumulator chains in e
 // balanced(x): |x| <= 2^40 -> a representative with |r| <= p/2 + 1 (int32).
 // Forward butterflies grow |x| by at most 0.51 p per level, inverse (GS) sums
 // at most double it: eight levels from |x| < p stay below 2^39.
+// Larger inputs: for |x| < 2^48 the same steps stay exact (|h / p| < 2^49,
+// the quotient off by at most 2^-4, |l| <= 2^26), with |r| <= 0.63 p, and
+// balanced() still returns |r| <= p/2 + 1.  The fused encode relies on this:
+// it feeds its unreduced dot sums (u < 2^47) straight into the forward
+// column levels, which then stay below 2^48.
 // ---------------------------------------------------------------------------
 constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
 
@@ -701,14 +706,9 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
             const uint32_t pt = 2 * grp + u;
             if (pt < t.PTenc) {
                 const v4i (&A)[KS] = A2[u];
-                double off[4], inv[4], pd[4];
+                double off[4];
 #pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const EncPrime c = cst[min(pt * 8 + 2 * g + kh, t.P - 1)];
-                    off[g] = c.off;
-                    inv[g] = c.inv;
-                    pd[g] = double(c.p);
-                }
+                for (int g = 0; g < 4; ++g) off[g] = cst[min(pt * 8 + 2 * g + kh, t.P - 1)].off;
                 v16i acc[2] = {v16i{}, v16i{}};
 #pragma unroll
                 for (int s2 = 0; s2 < KS; ++s2) {
@@ -723,10 +723,9 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
                     for (int g = 0; g < 4; ++g) {
                         const int32_t lo = acc[c][4 * g] + acc[c][4 * g + 1] * 256;
                         const int32_t hi = acc[c][4 * g + 2] + acc[c][4 * g + 3] * 256;
-                        const double ud = fma(double(hi), 65536.0, double(lo)) + off[g];
-                        const double qd = __builtin_trunc(ud * inv[g]);
-                        const uint32_t rr = uint32_t(fma(-qd, pd[g], ud));
-                        X[(8 * u + 2 * g + kh) * CPW * S + xrow] = double(min(rr, rr - uint32_t(pd[g])));
+                        // u = x (mod p) in [0, 2^47), exact; the transform's mul_mod and
+                        // sums stay exact on it (|u w / p| < 2^48), so no reduction here
+                        X[(8 * u + 2 * g + kh) * CPW * S + xrow] = fma(double(hi), 65536.0, double(lo)) + off[g];
                     }
                 }
             }
