@@ -111,6 +111,14 @@ def test_product_random_modulus(pkg, orc, gu, N, W):
     product_case(pkg, orc, gu, N, W, pkg.datagen.modulus(W, 17), seed=5, edge_fraction=0.1)
 
 
+@pytest.mark.parametrize("N,W", [(32768, 70), (65536, 100), (65536, 1955)])
+def test_product_extreme_widths(pkg, orc, gu, N, W):
+    """Large N with the narrowest and widest moduli: few primes (a single
+    encode weight tile, one prime pair in the fused kernel) and the most
+    (127 primes, the widest CRT GEMM)."""
+    product_case(pkg, orc, gu, N, W, pkg.datagen.modulus(W, 23), seed=7, edge_fraction=0.1)
+
+
 @pytest.mark.parametrize("gap", [1, (1 << 32) - 1])
 def test_product_near_power_modulus(pkg, orc, gu, gap):
     product_case(pkg, orc, gu, 65536, 1747, (1 << 1747) - gap, seed=9)
