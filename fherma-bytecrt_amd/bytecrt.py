@@ -156,7 +156,7 @@ class Engine:
         if rc != BCRT_OK:
             _raise(rc, err.value.decode())
         self._h = handle
-        self.pipeline = (16, 4)  # the library's default (chunk, streams)
+        self.pipeline = (32, 2)  # the library's default (chunk, streams)
         self.encode_tile = min(5, (N // 256).bit_length() - 1)  # the library's default ts_log
         self.fuse_min_tiles = 256  # the library's default fusion threshold
         g = np.zeros(10, dtype=np.uint32)

@@ -50,8 +50,8 @@ struct bcrt_engine {
     std::vector<hipStream_t> workers;
     std::vector<hipEvent_t> worker_done;
     hipEvent_t fork = nullptr;
-    uint32_t pipe_chunk = 16;
-    uint32_t pipe_streams = 4;
+    uint32_t pipe_chunk = 32;  // near the best of every B=128 sweep; tune.py records per geometry
+    uint32_t pipe_streams = 2;
     std::string last_error;
 };
 

@@ -78,7 +78,7 @@ int bcrt_run_device_graph(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d
  * product slots on three streams, native-batched/solve.cu:17-18,56-156):
  * batches larger than `chunk` products are split into chunks that run on
  * `streams` worker streams (1..16, default priority), forked from and joined
- * back into the caller's stream.  Defaults: chunk 16, streams 4.  Results do
+ * back into the caller's stream.  Defaults: chunk 32, streams 2.  Results do
  * not depend on the setting. */
 int bcrt_set_pipeline(bcrt_engine* e, uint32_t chunk, uint32_t streams);
 
