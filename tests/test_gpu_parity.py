@@ -111,6 +111,27 @@ def test_product_random_modulus(pkg, orc, gu, N, W):
     product_case(pkg, orc, gu, N, W, pkg.datagen.modulus(W, 17), seed=5, edge_fraction=0.1)
 
 
+@pytest.mark.parametrize("N,W", [(4096, 1000), (8192, 500), (16384, 129), (16384, 1200),
+                                 (32768, 1500), (65536, 640)])
+def test_batch_off_diagonal_geometries(pkg, orc, gu, N, W):
+    """Geometries away from the configs' W ~ N / 37.5 line (other encode and
+    CRT k-step counts, fused and separate first stages): a batch of two
+    through the device path, each product against the oracle."""
+    B = 2
+    q = pkg.datagen.modulus(W, 31)
+    e = pkg.Engine(N, W, q, max_batch=B)
+    a = np.stack([pkg.datagen.operands(N, q, 1400 + i, edges_first=(i == 0), edge_fraction=0.1) for i in range(B)])
+    b = np.stack([pkg.datagen.operands(N, q, 1500 + i, edge_fraction=0.1) for i in range(B)])
+    out = gu.empty_dev(B * N * e.L)
+    e.run_device(gu.to_dev(a), gu.to_dev(b), out, B)
+    got = gu.from_dev(out).reshape(B, N, e.L)
+    plan = orc.Plan(N, W, q)
+    for i in range(B):
+        want = plan.product(a[i], b[i])
+        bad = np.argwhere((got[i] != want).any(axis=1))
+        assert bad.size == 0, f"product {i}: {len(bad)} mismatches, first {bad[:4].ravel().tolist()}"
+
+
 @pytest.mark.parametrize("N,W", [(32768, 70), (65536, 100), (65536, 1955)])
 def test_product_extreme_widths(pkg, orc, gu, N, W):
     """Large N with the narrowest and widest moduli: few primes (a single
