@@ -644,7 +644,11 @@ constexpr uint32_t kEfS = exchange_stride<256>();
 
 size_t ef_smem_bytes(const DevTables& t) {
     const size_t stage = size_t(kEncCoefs) * (t.Kenc + 16), exch = size_t(16) * kEfS * sizeof(double);
-    return std::max(stage, exch) + sizeof(EncPrime) * kMaxPrimesCrt;
+    // + one pair of weight tiles (64 rows of Kenc + 16 bytes) when Kenc <= 224;
+    // at Kenc = 256 that would leave room for only two workgroups per CU, so
+    // that width reads its weight fragments from global memory instead
+    const size_t wtiles = t.Kenc <= 224 ? size_t(64) * (t.Kenc + 16) : 0;
+    return std::max(stage, exch) + sizeof(EncPrime) * kMaxPrimesCrt + wtiles;
 }
 
 template <int KS, int NLOG1>
@@ -659,6 +663,22 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
     const uint32_t r0 = bx * CPW;
     const size_t stage = size_t(kEncCoefs) * (t.Kenc + 16), exch = size_t(16) * kEfS * sizeof(double);
     EncPrime* cst = reinterpret_cast<EncPrime*>(smem + (stage > exch ? stage : exch));
+    // Each pair's two weight tiles are copied into LDS once per workgroup (the
+    // four waves then read their fragments there, instead of each loading
+    // them from L2), the next pair's during the current transform.
+    constexpr bool kStageW = KS <= 7;
+    uint8_t* wbuf = reinterpret_cast<uint8_t*>(cst + kMaxPrimesCrt);  // weight tiles of the current pair
+    const uint32_t wstride = t.Kenc + 16;
+    // copies weight tiles 2 grp, 2 grp + 1 (contiguous in wenc) into wbuf as 16-byte words
+    auto load_weights = [&](uint32_t grp) {
+        if constexpr (!kStageW) return;
+        const uint32_t rows = min(64u, (t.PTenc - 2 * grp) * 32), wpr = t.Kenc / 16;
+        const uint4* src = reinterpret_cast<const uint4*>(t.wenc + size_t(2 * grp) * 32 * t.Kenc);
+        for (uint32_t i = threadIdx.x; i < rows * wpr; i += kThreads) {
+            const uint32_t rw = i / wpr, cw = i - rw * wpr;
+            *reinterpret_cast<uint4*>(wbuf + rw * wstride + 16 * cw) = src[i];
+        }
+    };
     for (uint32_t j = threadIdx.x; j < t.P; j += kThreads)
         cst[j] = reinterpret_cast<const EncPrime*>(t.enc_prime)[j];
     const uint32_t* in = (op ? b : a) + size_t(prod) * t.N * t.L;
@@ -686,26 +706,26 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
                                                              32 * s2);
         }
     }
-    __syncthreads();  // the staging area becomes the exchange
+    load_weights(0);
+    __syncthreads();  // the staging area becomes the exchange; the first pair's weights are in
     double* X = reinterpret_cast<double*>(smem);
     const uint32_t q = threadIdx.x / T, jt = q / CPW, cq = q % CPW, tau = threadIdx.x % T;
     const uint32_t npairs = (t.PTenc + 1) / 2;
     for (uint32_t grp = 0; grp < npairs; ++grp) {
-        // both tiles' weight fragments up front: the second tile's loads land
-        // while the first one computes (a missing last tile reloads the one before)
-        v4i A2[2][KS];
-#pragma unroll
-        for (uint32_t u = 0; u < 2; ++u) {
-            const uint32_t pt = min(2 * grp + u, t.PTenc - 1);
-            const int8_t* wrow = t.wenc + (size_t(pt) * 32 + row) * t.Kenc + 16 * kh;
-#pragma unroll
-            for (int s2 = 0; s2 < KS; ++s2) A2[u][s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
-        }
 #pragma unroll
         for (uint32_t u = 0; u < 2; ++u) {
             const uint32_t pt = 2 * grp + u;
             if (pt < t.PTenc) {
-                const v4i (&A)[KS] = A2[u];
+                v4i A[KS];
+                if constexpr (kStageW) {
+                    const uint8_t* wrow = wbuf + (u * 32 + row) * wstride + 16 * kh;
+#pragma unroll
+                    for (int s2 = 0; s2 < KS; ++s2) A[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
+                } else {
+                    const int8_t* wrow = t.wenc + (size_t(pt) * 32 + row) * t.Kenc + 16 * kh;
+#pragma unroll
+                    for (int s2 = 0; s2 < KS; ++s2) A[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
+                }
                 double off[4];
 #pragma unroll
                 for (int g = 0; g < 4; ++g) off[g] = cst[min(pt * 8 + 2 * g + kh, t.P - 1)].off;
@@ -730,7 +750,8 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
                 }
             }
         }
-        __syncthreads();
+        __syncthreads();  // every wave is done with this pair's weights
+        if (grp + 1 < npairs) load_weights(grp + 1);  // lands during the transform
         // forward column levels 0..n1-1 of prime j, column r0 + cq (k_fwd_cols)
         const uint32_t j = 16 * grp + jt;
         double* xc = X + q * S;
@@ -1500,9 +1521,11 @@ hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipS
 
 hipError_t configure_kernels() {
     // Every kernel's dynamic LDS stays within the 64 KiB default (the largest,
-    // encode with Kenc = 256, takes 37.9 KB; the CRT at most 35.8 KB), so no
-    // attribute needs raising.
+    // the fused encode at Kenc = 224 with its staged weight tiles, takes 52 KB;
+    // the CRT at most 35.8 KB), so no attribute needs raising.
     static_assert(size_t(kEncCoefs) * (256 + 16) + sizeof(EncPrime) * kMaxPrimesCrt <= 65536, "encode LDS");
+    static_assert(size_t(16) * kEfS * sizeof(double) + sizeof(EncPrime) * kMaxPrimesCrt + 64 * (224 + 16) <= 65536,
+                  "fused encode LDS");
     static_assert(size_t(kCrtCoefs) * (32 * 17 + 16) <= 65536 && size_t(65) * kCrtCoefs * 8 <= 65536, "CRT LDS");
     return hipSuccess;
 }
