@@ -145,8 +145,11 @@ def test_product_near_power_modulus(pkg, orc, gu, gap):
     product_case(pkg, orc, gu, 65536, 1747, (1 << 1747) - gap, seed=9)
 
 
-def test_product_closed_forms(pkg, orc, gu):
-    N, W = 16384, 436
+@pytest.mark.parametrize("N,W", [(16384, 436), (65536, 1747)])
+def test_product_closed_forms(pkg, orc, gu, N, W):
+    """Products whose result has a closed form.  At N=65,536 the single product
+    takes the fused encode (512 workgroups) and its partly filled last group of
+    16 primes (P=114)."""
     q = pkg.datagen.modulus(W, 21)
     e = pkg.Engine(N, W, q)
     dg = pkg.datagen
@@ -173,6 +176,14 @@ def test_product_closed_forms(pkg, orc, gu):
     f1 = dg.filled(N, q, q - 1)
     res = orc.rows_to_ints(run(f1, f1))
     assert res == [(2 * k + 2 - N) % q for k in range(N)]
+    # the largest canonical magnitudes: h = floor(q/2) as a constant and dense
+    h = q // 2
+    hc = dg.constant(N, q, h)
+    res = orc.rows_to_ints(run(hc, hc))
+    assert res[0] == h * h % q and not any(res[1:])
+    fh = dg.filled(N, q, h)
+    res = orc.rows_to_ints(run(fh, fh))
+    assert res == [h * h * (2 * k + 2 - N) % q for k in range(N)]
 
 
 def test_batch_graph_and_whole_call(pkg, orc, gu):
