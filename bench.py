@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--sweep", action="store_true",
                     help="autotune the batch pipelining, write the pipeline table and exit")
-    ap.add_argument("--sweep-out", default="gpurun_out/pipeline_table.json")
+    ap.add_argument("--sweep-out", default="pipeline_table.json",
+                    help="where --sweep writes its table (copy it over fherma-bytecrt_amd/pipeline_table.json to use it)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="profiling mode: device-to-device copy of this many MiB, --reps times")
     return ap.parse_args()
