@@ -712,6 +712,10 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
     const uint32_t q = threadIdx.x / T, jt = q / CPW, cq = q % CPW, tau = threadIdx.x % T;
     const uint32_t npairs = (t.PTenc + 1) / 2;
     for (uint32_t grp = 0; grp < npairs; ++grp) {
+        // MFMA and VALU instructions share a SIMD's issue port: a wave in its
+        // MFMA block goes first, so the matrix pipe stays fed while the other
+        // resident workgroups' transforms fill the gaps (measured 1.5% faster)
+        __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (uint32_t u = 0; u < 2; ++u) {
             const uint32_t pt = 2 * grp + u;
@@ -750,6 +754,7 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
                 }
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();  // every wave is done with this pair's weights
         if (grp + 1 < npairs) load_weights(grp + 1);  // lands during the transform
         // forward column levels 0..n1-1 of prime j, column r0 + cq (k_fwd_cols)
