@@ -8,6 +8,7 @@ Outputs (git-ignored, shipped in-tree to the GPU host):
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -20,8 +21,8 @@ LIB = PKG / "lib"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_io.cpp", "bytecrt_trace.cpp", "bytecrt_engine.cpp",
-           "bytecrt_server.cpp", "fherma_solve.cpp"]
+SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_io.cpp", "bytecrt_trace.cpp",
+           "bytecrt_engine.cpp", "bytecrt_server.cpp", "fherma_solve.cpp"]
 # harness_main.cpp is linked separately into lib/bytecrt_solution.
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
@@ -41,7 +42,7 @@ def _run(cmd):
 
 def build(verbose: bool = False) -> Path:
     LIB.mkdir(exist_ok=True)
-    objs = []
+    jobs = []
     for src in SOURCES:
         s = CSRC / src
         o = LIB / (s.stem + ".o")
@@ -50,8 +51,12 @@ def build(verbose: bool = False) -> Path:
             cmd[1:1] = DEVICE
         else:
             cmd[1:1] = [f"--offload-arch={ARCH}", "-x", "hip"] if src in ("bytecrt_engine.cpp", "bytecrt_server.cpp") else []
-        _run(cmd)
-        objs.append(str(o))
+        jobs.append((cmd, str(o)))
+    # Independent translation units: compile them side by side.
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 8))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        list(pool.map(_run, [cmd for cmd, _ in jobs]))
+    objs = [o for _, o in jobs]
     out = LIB / "libbytecrt.so"
     tmp = LIB / "libbytecrt.so.tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs,

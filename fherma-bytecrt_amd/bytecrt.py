@@ -253,7 +253,9 @@ class Mix:
         arr = (_Group * len(groups))()
         for i, (eng, a, b, c, batch) in enumerate(groups):
             arr[i] = _Group(eng._h.value, _dptr(a), _dptr(b), _dptr(c), batch)
-        self._engines = [g[0] for g in groups]  # keep the engines alive
+        # The library defers an engine's destruction while a mix holds it; the
+        # references keep the Python objects (and their handles) alive too.
+        self._engines = [g[0] for g in groups]
         handle = ctypes.c_void_p()
         err = ctypes.create_string_buffer(512)
         rc = lib.bcrt_mix_create(ctypes.byref(handle), arr, len(groups), err, 512)
@@ -327,9 +329,10 @@ class Server:
         return c, lat.value
 
     def stats(self) -> dict:
-        out = np.zeros(4, dtype=np.uint64)
+        out = np.zeros(8, dtype=np.uint64)
         self._check(load_library().bcrt_server_stats(self._h, out.ctypes.data_as(_u64p)))
-        return dict(completed=int(out[0]), waves=int(out[1]), max_wave=int(out[2]), plans=int(out[3]))
+        keys = ("completed", "waves", "max_wave", "plans", "evictions", "table_bytes", "table_cap", "arena_bytes")
+        return {k: int(v) for k, v in zip(keys, out)}
 
     def close(self):
         if getattr(self, "_h", None):

@@ -29,29 +29,46 @@ namespace bcrt {
 hipError_t configure_kernels();
 }
 
+// Fork/join events of one pipelined launch sequence.  Eager runs and each
+// captured graph use different sets: an event object is never recorded into a
+// capture while an eager record of it may still be pending.
+struct EventSet {
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
+};
+
 struct bcrt_engine {
     bcrt::HostPlan plan;
     bcrt::DevTables tables{};
-    uint32_t max_batch = 0;
+    uint32_t max_batch = 0;  // 0: tables only, no residue-plane workspace
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> allocations;
+    size_t table_bytes = 0;      // device bytes of the plan tables
     uint32_t* planes = nullptr;  // [max_batch][2][P][N]
     // Whole-call host I/O (pinned staging, copy pools, drain thread), created
     // on the first whole call.
     std::unique_ptr<bcrt::HostIo> hostio;
-    // Captured graph (one key).
+    // Captured graph (one key) with the events its capture recorded, and an
+    // event recorded behind every launch of it so it is destroyed only once
+    // its last launch has finished.
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
+    EventSet graph_events;
+    hipEvent_t graph_done = nullptr;
+    bool graph_launched = false;
     const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
     uint32_t graph_batch = 0;
     // Batch pipelining: chunks of `pipe_chunk` products round-robin over worker
     // streams so that stages of different chunks overlap on the device.
     std::vector<hipStream_t> workers;
-    std::vector<hipEvent_t> worker_done;
-    hipEvent_t fork = nullptr;
+    EventSet eager_events;
     uint32_t pipe_chunk = 32;  // near the best of every B=128 sweep; tune.py records per geometry
     uint32_t pipe_streams = 2;
+    // Mixed batches that hold this engine's tables and workspace; while any
+    // does, bcrt_destroy only marks the engine and the last mix frees it.
+    uint32_t mix_refs = 0;
+    bool destroy_pending = false;
     std::string last_error;
 };
 
@@ -70,6 +87,7 @@ T* upload(bcrt_engine& s, const std::vector<T>& v) {
     void* p = nullptr;
     check(hipMalloc(&p, v.size() * sizeof(T)), "allocate table");
     s.allocations.push_back(p);
+    s.table_bytes += v.size() * sizeof(T);
     check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload table");
     return static_cast<T*>(p);
 }
@@ -103,23 +121,51 @@ void require_batch(const bcrt_engine* e, uint32_t batch) {
                                     std::to_string(e->max_batch) + "]");
 }
 
+void destroy_events(EventSet& ev) {
+    for (auto x : ev.joins) (void)hipEventDestroy(x);
+    ev.joins.clear();
+    if (ev.fork) (void)hipEventDestroy(ev.fork);
+    ev.fork = nullptr;
+}
+
+// Grows an event set to `n` join events (and its fork event).
+void ensure_events(EventSet& ev, size_t n) {
+    if (!ev.fork) check(hipEventCreateWithFlags(&ev.fork, hipEventDisableTiming), "create fork event");
+    while (ev.joins.size() < n) {
+        hipEvent_t x = nullptr;
+        check(hipEventCreateWithFlags(&x, hipEventDisableTiming), "create join event");
+        ev.joins.push_back(x);
+    }
+}
+
+// Waits for the last launch of the captured graph, then destroys it with the
+// events its capture recorded.
 void drop_graph(bcrt_engine* e) {
+    if (e->graph_launched) (void)hipEventSynchronize(e->graph_done);
+    e->graph_launched = false;
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
+    destroy_events(e->graph_events);
     e->graph_exec = nullptr;
     e->graph = nullptr;
     e->graph_batch = 0;
 }
 
+// Waits for everything the engine enqueued on its own streams.
+void quiesce(bcrt_engine* e) {
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto st : e->workers) (void)hipStreamSynchronize(st);
+}
+
 void release(bcrt_engine* e) {
     drop_graph(e);
     e->hostio.reset();
-    for (auto ev : e->worker_done) (void)hipEventDestroy(ev);
-    e->worker_done.clear();
+    quiesce(e);
+    destroy_events(e->eager_events);
     for (auto st : e->workers) (void)hipStreamDestroy(st);
     e->workers.clear();
-    if (e->fork) (void)hipEventDestroy(e->fork);
-    e->fork = nullptr;
+    if (e->graph_done) (void)hipEventDestroy(e->graph_done);
+    e->graph_done = nullptr;
     for (void* p : e->allocations) (void)hipFree(p);
     e->allocations.clear();
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -131,20 +177,20 @@ void ensure_workers(bcrt_engine* e) {
         hipStream_t st = nullptr;
         check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "create worker stream");
         e->workers.push_back(st);
-        hipEvent_t ev = nullptr;
-        check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "create worker event");
-        e->worker_done.push_back(ev);
     }
-    if (!e->fork) check(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming), "create fork event");
 }
+
+bool pipelined(const bcrt_engine* e, uint32_t batch) { return batch > e->pipe_chunk && e->pipe_streams > 1; }
 
 // Enqueue the product for a batch on `origin`: one launch sequence when the
 // batch fits a chunk, else chunks spread over the worker streams, forked from
-// and joined back into `origin` (also valid inside a stream capture).
+// and joined back into `origin` with the events of `ev` (also valid inside a
+// stream capture).
 void enqueue_batch(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
-                   uint32_t batch, hipStream_t origin) {
+                   uint32_t batch, hipStream_t origin, EventSet& ev) {
     const auto& g = e->plan.g;
-    if (batch <= e->pipe_chunk || e->pipe_streams <= 1) {
+    if (!e->planes) throw std::invalid_argument("ByteCRT engine has no residue-plane workspace");
+    if (!pipelined(e, batch)) {
         const bcrt::BatchArgs args{d_a, d_b, d_c, e->planes, batch};
         check(bcrt::launch_product(e->tables, args, origin), "launch product");
         return;
@@ -152,8 +198,9 @@ void enqueue_batch(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uin
     ensure_workers(e);
     const size_t io = size_t(g.N) * g.L, pl = size_t(2) * g.P * g.N;
     const uint32_t used = std::min<uint32_t>(e->pipe_streams, (batch + e->pipe_chunk - 1) / e->pipe_chunk);
-    check(hipEventRecord(e->fork, origin), "fork");
-    for (uint32_t k = 0; k < used; ++k) check(hipStreamWaitEvent(e->workers[k], e->fork, 0), "fork wait");
+    ensure_events(ev, used);
+    check(hipEventRecord(ev.fork, origin), "fork");
+    for (uint32_t k = 0; k < used; ++k) check(hipStreamWaitEvent(e->workers[k], ev.fork, 0), "fork wait");
     uint32_t i = 0;
     for (uint32_t c0 = 0; c0 < batch; c0 += e->pipe_chunk, ++i) {
         const uint32_t n = std::min(e->pipe_chunk, batch - c0);
@@ -161,8 +208,8 @@ void enqueue_batch(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uin
         check(bcrt::launch_product(e->tables, args, e->workers[i % used]), "launch chunk");
     }
     for (uint32_t k = 0; k < used; ++k) {
-        check(hipEventRecord(e->worker_done[k], e->workers[k]), "join record");
-        check(hipStreamWaitEvent(origin, e->worker_done[k], 0), "join wait");
+        check(hipEventRecord(ev.joins[k], e->workers[k]), "join record");
+        check(hipStreamWaitEvent(origin, ev.joins[k], 0), "join wait");
     }
 }
 
@@ -180,22 +227,27 @@ int run_whole(bcrt_engine* e, const uint32_t* a, const uint32_t* b, uint32_t* c,
         if (!e->hostio) e->hostio = std::make_unique<bcrt::HostIo>(size_t(g.N) * g.L, e->max_batch, e->stream);
         e->hostio->run(a, b, batch, c, sink, ctx,
                        [e](const uint32_t* da, const uint32_t* db, uint32_t* dc, uint32_t n, hipStream_t st) {
-                           enqueue_batch(e, da, db, dc, n, st);
+                           enqueue_batch(e, da, db, dc, n, st, e->eager_events);
                        });
     });
 }
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int bcrt_create(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, uint32_t L,
-                const uint32_t* q, char* err, size_t errlen) {
+// Global lock for the engine <-> mix reference counts.
+std::mutex g_refs;
+
+// Builds an engine: plan, device tables and, when max_batch > 0, the residue-
+// plane workspace for max_batch products (max_batch 0: tables only, for the
+// server, whose requests bring their planes from its arena).
+int create_engine(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, uint32_t L, const uint32_t* q,
+                  char* err, size_t errlen) {
     if (!out) return BCRT_EINVAL;
     *out = nullptr;
     auto* e = new bcrt_engine();
     const int rc = guarded(e, [&] {
-        if (max_batch == 0) throw std::invalid_argument("ByteCRT max_batch must be positive");
         e->plan = bcrt::make_plan(N, W, L, q);
         const auto& g = e->plan.g;
         e->max_batch = max_batch;
@@ -204,6 +256,7 @@ int bcrt_create(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, u
         // A blocking stream: work enqueued here is ordered with the legacy
         // default stream, which is where callers' allocations and copies land.
         check(hipStreamCreate(&e->stream), "create stream");
+        check(hipEventCreateWithFlags(&e->graph_done, hipEventDisableTiming), "create graph event");
         bcrt::DevTables& t = e->tables;
         t.N = g.N;
         t.W = g.W;
@@ -237,10 +290,12 @@ int bcrt_create(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, u
         // The host copies of the big tables are no longer needed.
         std::vector<uint32_t>().swap(e->plan.zf);
         std::vector<uint32_t>().swap(e->plan.zi);
-        void* p = nullptr;
-        check(hipMalloc(&p, size_t(max_batch) * 2 * g.P * g.N * sizeof(uint32_t)), "allocate residue planes");
-        e->allocations.push_back(p);
-        e->planes = static_cast<uint32_t*>(p);
+        if (max_batch > 0) {
+            void* p = nullptr;
+            check(hipMalloc(&p, size_t(max_batch) * 2 * g.P * g.N * sizeof(uint32_t)), "allocate residue planes");
+            e->allocations.push_back(p);
+            e->planes = static_cast<uint32_t*>(p);
+        }
     });
     if (rc != BCRT_OK) {
         if (err && errlen) {
@@ -254,8 +309,69 @@ int bcrt_create(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, u
     return BCRT_OK;
 }
 
+// Captures the batch's launch sequence into the engine's graph, keyed by
+// (pointers, batch).  Nothing the engine enqueued earlier is still pending
+// when the capture starts, and the capture records its own fork/join events.
+void capture_graph(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch) {
+    drop_graph(e);
+    if (pipelined(e, batch)) ensure_workers(e);
+    quiesce(e);
+    check(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    std::string failure;
+    try {
+        enqueue_batch(e, d_a, d_b, d_c, batch, e->stream, e->graph_events);
+    } catch (const std::exception& x) {
+        failure = x.what();
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (!failure.empty()) {
+        if (g) (void)hipGraphDestroy(g);
+        throw std::runtime_error(failure + " (during capture)");
+    }
+    check(ce, "end capture");
+    e->graph = g;
+    check(hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0), "instantiate graph");
+    e->graph_key[0] = d_a;
+    e->graph_key[1] = d_b;
+    e->graph_key[2] = d_c;
+    e->graph_batch = batch;
+}
+
+}  // namespace
+
+namespace bcrt {
+
+int create_tables_only(bcrt_engine** out, uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, char* err,
+                       size_t errlen) {
+    return create_engine(out, 0, N, W, L, q, err, errlen);
+}
+
+size_t engine_table_bytes(const bcrt_engine* e) { return e->table_bytes; }
+
+}  // namespace bcrt
+
+extern "C" {
+
+int bcrt_create(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W, uint32_t L,
+                const uint32_t* q, char* err, size_t errlen) {
+    if (out) *out = nullptr;
+    if (out && max_batch == 0) {
+        if (err && errlen) std::snprintf(err, errlen, "%s", "ByteCRT max_batch must be positive");
+        return BCRT_EINVAL;
+    }
+    return create_engine(out, max_batch, N, W, L, q, err, errlen);
+}
+
 void bcrt_destroy(bcrt_engine* e) {
     if (!e) return;
+    {
+        std::lock_guard<std::mutex> g(g_refs);
+        if (e->mix_refs > 0) {
+            e->destroy_pending = true;  // the last mix that holds it frees it
+            return;
+        }
+    }
     release(e);
     delete e;
 }
@@ -266,7 +382,7 @@ int bcrt_run_device(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, ui
     return guarded(e, [&] {
         require_batch(e, batch);
         if (!d_a || !d_b || !d_c) throw std::invalid_argument("ByteCRT null device pointer");
-        enqueue_batch(e, d_a, d_b, d_c, batch, pick(e, stream));
+        enqueue_batch(e, d_a, d_b, d_c, batch, pick(e, stream), e->eager_events);
     });
 }
 
@@ -279,32 +395,11 @@ int bcrt_run_device_graph(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d
         hipStream_t s = pick(e, stream);
         const bool hit = e->graph_exec && e->graph_batch == batch && e->graph_key[0] == d_a &&
                          e->graph_key[1] == d_b && e->graph_key[2] == d_c;
-        if (!hit) {
-            drop_graph(e);
-            // Capture on the engine's own stream, then launch on the caller's.
-            if (batch > e->pipe_chunk && e->pipe_streams > 1) ensure_workers(e);
-            check(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-            std::string failure;
-            try {
-                enqueue_batch(e, d_a, d_b, d_c, batch, e->stream);
-            } catch (const std::exception& x) {
-                failure = x.what();
-            }
-            hipGraph_t g = nullptr;
-            const hipError_t ce = hipStreamEndCapture(e->stream, &g);
-            if (!failure.empty()) {
-                if (g) (void)hipGraphDestroy(g);
-                throw std::runtime_error(failure + " (during capture)");
-            }
-            check(ce, "end capture");
-            e->graph = g;
-            check(hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0), "instantiate graph");
-            e->graph_key[0] = d_a;
-            e->graph_key[1] = d_b;
-            e->graph_key[2] = d_c;
-            e->graph_batch = batch;
-        }
+        // Capture on the engine's own stream, then launch on the caller's.
+        if (!hit) capture_graph(e, d_a, d_b, d_c, batch);
         check(hipGraphLaunch(e->graph_exec, s), "graph launch");
+        check(hipEventRecord(e->graph_done, s), "record graph launch");
+        e->graph_launched = true;
     });
 }
 
@@ -350,6 +445,7 @@ int bcrt_run_stages(bcrt_engine* e, uint32_t stage_mask, const uint32_t* d_a, co
         require_batch(e, batch);
         if ((stage_mask & 1) && (!d_a || !d_b)) throw std::invalid_argument("ByteCRT encode needs a and b");
         if ((stage_mask & 16) && !d_c) throw std::invalid_argument("ByteCRT CRT needs c");
+        if (!d_planes && !e->planes) throw std::invalid_argument("ByteCRT engine has no residue-plane workspace");
         const bcrt::BatchArgs args{d_a, d_b, d_c, d_planes ? d_planes : e->planes, batch};
         hipStream_t s = pick(e, stream);
         if ((stage_mask & 3) == 3) {
@@ -442,10 +538,12 @@ constexpr uint32_t kMixChunkMin = 8;  // products per chunk before splitting fur
 
 struct bcrt_mix {
     std::vector<std::vector<bcrt::GroupDesc>> chunks;
+    std::vector<bcrt_engine*> engines;  // referenced (mix_refs) for the mix's lifetime
     hipStream_t stream = nullptr;
+    hipEvent_t graph_done = nullptr;  // recorded behind every graph launch
+    bool graph_launched = false;
     std::vector<hipStream_t> workers;
-    std::vector<hipEvent_t> joins;
-    hipEvent_t fork = nullptr;
+    EventSet eager_events, graph_events;  // fork/join events: eager runs, the capture
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::string last_error;
@@ -453,36 +551,42 @@ struct bcrt_mix {
 
 namespace {
 
+// Waits for the mix's last launches, then frees its graph, events and streams.
 void release_mix(bcrt_mix* m) {
+    if (m->graph_launched) (void)hipEventSynchronize(m->graph_done);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (auto st : m->workers) (void)hipStreamSynchronize(st);
+    if (m->graph_done) (void)hipEventDestroy(m->graph_done);
+    m->graph_done = nullptr;
+    m->graph_launched = false;
     if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
-    for (auto ev : m->joins) (void)hipEventDestroy(ev);
+    destroy_events(m->eager_events);
+    destroy_events(m->graph_events);
     for (auto st : m->workers) (void)hipStreamDestroy(st);
-    if (m->fork) (void)hipEventDestroy(m->fork);
     if (m->stream) (void)hipStreamDestroy(m->stream);
-    m->joins.clear();
     m->workers.clear();
-    m->fork = nullptr;
     m->graph_exec = nullptr;
     m->graph = nullptr;
     m->stream = nullptr;
 }
 
-// Enqueue every chunk on `origin` (forked over the worker streams when there
-// are several; also valid inside a stream capture).
-void enqueue_mix(bcrt_mix* m, hipStream_t origin) {
+// Enqueue every chunk on `origin` (forked over the worker streams with the
+// events of `ev` when there are several; also valid inside a stream capture).
+void enqueue_mix(bcrt_mix* m, hipStream_t origin, EventSet& ev) {
     if (m->chunks.size() == 1) {
         check(bcrt::launch_product_groups(m->chunks[0].data(), uint32_t(m->chunks[0].size()), origin),
               "launch mixed batch");
         return;
     }
-    check(hipEventRecord(m->fork, origin), "fork");
+    ensure_events(ev, m->chunks.size());
+    check(hipEventRecord(ev.fork, origin), "fork");
     for (size_t c = 0; c < m->chunks.size(); ++c) {
-        check(hipStreamWaitEvent(m->workers[c], m->fork, 0), "fork wait");
+        check(hipStreamWaitEvent(m->workers[c], ev.fork, 0), "fork wait");
         check(bcrt::launch_product_groups(m->chunks[c].data(), uint32_t(m->chunks[c].size()), m->workers[c]),
               "launch mixed chunk");
-        check(hipEventRecord(m->joins[c], m->workers[c]), "join record");
-        check(hipStreamWaitEvent(origin, m->joins[c], 0), "join wait");
+        check(hipEventRecord(ev.joins[c], m->workers[c]), "join record");
+        check(hipStreamWaitEvent(origin, ev.joins[c], 0), "join wait");
     }
 }
 
@@ -554,16 +658,15 @@ extern "C" int bcrt_mix_create(bcrt_mix** out, const bcrt_group* groups, uint32_
         }
         const uint32_t nchunks = uint32_t(m->chunks.size());
         check(hipStreamCreate(&m->stream), "create mix stream");
+        check(hipEventCreateWithFlags(&m->graph_done, hipEventDisableTiming), "create graph event");
         if (nchunks > 1) {
-            check(hipEventCreateWithFlags(&m->fork, hipEventDisableTiming), "create fork event");
             for (uint32_t c = 0; c < nchunks; ++c) {
                 hipStream_t st = nullptr;
                 check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "create mix worker stream");
                 m->workers.push_back(st);
-                hipEvent_t ev = nullptr;
-                check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "create join event");
-                m->joins.push_back(ev);
             }
+            ensure_events(m->eager_events, nchunks);
+            ensure_events(m->graph_events, nchunks);
         }
     } catch (const std::invalid_argument& x) {
         rc = BCRT_EINVAL;
@@ -578,6 +681,15 @@ extern "C" int bcrt_mix_create(bcrt_mix** out, const bcrt_group* groups, uint32_
         delete m;
         return rc;
     }
+    {
+        // The mix's launches read its engines' tables and planes: keep them
+        // alive until the mix is destroyed.
+        std::lock_guard<std::mutex> g(g_refs);
+        for (uint32_t k = 0; k < ngroups; ++k) {
+            m->engines.push_back(groups[k].engine);
+            ++groups[k].engine->mix_refs;
+        }
+    }
     *out = m;
     return BCRT_OK;
 }
@@ -587,13 +699,16 @@ extern "C" int bcrt_mix_run(bcrt_mix* m, void* stream, int graph) {
     try {
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m->stream;
         if (!graph) {
-            enqueue_mix(m, s);
+            enqueue_mix(m, s, m->eager_events);
         } else {
             if (!m->graph_exec) {
+                // Nothing the mix enqueued earlier is pending when the capture starts.
+                (void)hipStreamSynchronize(m->stream);
+                for (auto st : m->workers) (void)hipStreamSynchronize(st);
                 check(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal), "begin capture");
                 std::string fail;
                 try {
-                    enqueue_mix(m, m->stream);
+                    enqueue_mix(m, m->stream, m->graph_events);
                 } catch (const std::exception& x) {
                     fail = x.what();
                 }
@@ -608,6 +723,8 @@ extern "C" int bcrt_mix_run(bcrt_mix* m, void* stream, int graph) {
                 check(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0), "instantiate graph");
             }
             check(hipGraphLaunch(m->graph_exec, s), "graph launch");
+            check(hipEventRecord(m->graph_done, s), "record graph launch");
+            m->graph_launched = true;
         }
         m->last_error.clear();
         return BCRT_OK;
@@ -622,6 +739,16 @@ extern "C" const char* bcrt_mix_last_error(const bcrt_mix* m) { return m ? m->la
 extern "C" void bcrt_mix_destroy(bcrt_mix* m) {
     if (!m) return;
     release_mix(m);
+    std::vector<bcrt_engine*> orphans;  // engines destroyed while this mix held them
+    {
+        std::lock_guard<std::mutex> g(g_refs);
+        for (bcrt_engine* e : m->engines)
+            if (--e->mix_refs == 0 && e->destroy_pending) orphans.push_back(e);
+    }
+    for (bcrt_engine* e : orphans) {
+        release(e);
+        delete e;
+    }
     delete m;
 }
 

@@ -2,9 +2,11 @@
 // time, are admitted into "waves" while the HBM budget, pinned staging and
 // in-flight slots allow, and each wave runs as one mixed launch per stage
 // (group kernels) on one of two compute streams, so a wave admitted while
-// another is on the GPU overlaps with it.  Plan tables stay resident per
-// (N, W, q); per-request device memory (operands, result, residue planes)
-// comes from one arena used as a FIFO ring.
+// another is on the GPU overlaps with it.  Plan tables (twiddles and MFMA
+// weights, no workspace) stay resident per (N, W, q) within a quarter of the
+// HBM budget, least recently used idle plans evicted first; per-request device
+// memory (operands, result, residue planes) comes from one arena, the other
+// three quarters, used as a FIFO ring.
 //
 // Two host threads: the admit thread forms waves, packs operands into pinned
 // staging (copy pool) and enqueues copies and kernels; the retire thread
@@ -35,6 +37,12 @@
 namespace bcrt {
 // Tables and geometry of an engine (bytecrt_engine.cpp).
 const DevTables& engine_tables(const bcrt_engine* e);
+// An engine with the plan's device tables and no residue-plane workspace.
+int create_tables_only(bcrt_engine** out, uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, char* err,
+                       size_t errlen);
+size_t engine_table_bytes(const bcrt_engine* e);
+// Primes of the plan for (N, W) (bytecrt_plan.cpp).
+uint32_t plan_prime_count(uint32_t N, uint32_t W);
 }
 
 namespace {
@@ -49,11 +57,22 @@ constexpr uint32_t kInflight = 4;  // waves on the device at once
 constexpr size_t kAlign = 256;
 
 struct Plan {
-    bcrt_engine* engine = nullptr;  // owns the resident tables
+    bcrt_engine* engine = nullptr;  // owns the resident tables (no workspace)
     bcrt::DevTables t{};
-    size_t io_words = 0, plane_words = 0;
+    size_t io_words = 0, plane_words = 0, table_bytes = 0;
+    uint32_t refs = 0;      // requests submitted and not yet retired
+    uint64_t last_use = 0;  // submit tick of the latest request
     ~Plan() { bcrt_destroy(engine); }
 };
+
+// Device bytes of a plan's tables before it is built: the two P x N twiddle
+// tables plus an upper bound on the small ones (primes, weights, digit tiles).
+size_t table_estimate(uint32_t N, uint32_t L, uint32_t P) {
+    const size_t Kenc = (4 * L + 31) / 32 * 32, Kcrt = (4 * (P + 1) + 31) / 32 * 32;
+    const size_t Mcrt = (4 * L + 1 + 31) / 32 * 32;
+    return size_t(2) * P * N * 4 + size_t(P) * 96 + (P + 7) / 8 * 32 * Kenc + Mcrt * Kcrt + Mcrt * 2 + L * 4 +
+           size_t(16) * 256;
+}
 
 struct Request {
     uint64_t id = 0;
@@ -83,13 +102,15 @@ struct Wave {
 
 size_t round_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
-size_t footprint(const Plan* p, int k) {
+size_t footprint(size_t io_words, size_t plane_words, int k) {
     switch (k) {
-        case kArena: return round_up((p->io_words * 3 + p->plane_words) * sizeof(uint32_t));
-        case kPinIn: return round_up(p->io_words * 2 * sizeof(uint32_t));
-        default: return round_up(p->io_words * sizeof(uint32_t));
+        case kArena: return round_up((io_words * 3 + plane_words) * sizeof(uint32_t));
+        case kPinIn: return round_up(io_words * 2 * sizeof(uint32_t));
+        default: return round_up(io_words * sizeof(uint32_t));
     }
 }
+
+size_t footprint(const Plan* p, int k) { return footprint(p->io_words, p->plane_words, k); }
 
 }  // namespace
 
@@ -109,7 +130,10 @@ struct bcrt_server {
     std::deque<std::unique_ptr<Wave>> inflight;      // same waves, oldest first
     std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
     std::map<std::string, std::unique_ptr<Plan>> plans;
-    std::mutex plan_m;
+    std::mutex plan_m;  // plans, table accounting; taken before m, never inside it
+    std::condition_variable plan_cv;  // a plan's last request retired
+    size_t table_cap = 0, table_used = 0;
+    uint64_t plan_tick = 0, evictions = 0;
     uint64_t next_id = 1;
     bool stop = false, admit_done = false;
     std::string fatal;
@@ -260,11 +284,82 @@ void bcrt_server::retire_loop() {
             ++completed;
         }
         if (!err.empty() && fatal.empty()) fatal = err;
+        std::vector<Plan*> used;
+        for (Request* r : w.reqs) used.push_back(r->plan);
         inflight.pop_front();
         sched->retire_oldest();
         done_cv.notify_all();
         admit_cv.notify_one();
+        lk.unlock();
+        {
+            // The wave's plans may be evicted once no request holds them.
+            std::lock_guard<std::mutex> pg(plan_m);
+            for (Plan* p : used) --p->refs;
+        }
+        plan_cv.notify_all();
+        lk.lock();
     }
+}
+
+// The plan for (q, N, W) with one more request holding it, built (and room
+// made for its tables by evicting idle plans, least recently used first) if
+// it is not resident.  Blocks while every plan that could make room is in
+// use.  Returns a status code; *out is the plan on success.
+static int acquire_plan(bcrt_server* s, uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, uint32_t P,
+                        Plan** out) {
+    std::unique_lock<std::mutex> pg(s->plan_m);
+    std::string key(reinterpret_cast<const char*>(q), size_t(L) * 4);
+    key += "|" + std::to_string(N) + "|" + std::to_string(W);
+    auto it = s->plans.find(key);
+    if (it == s->plans.end()) {
+        const size_t need = table_estimate(N, L, P);
+        if (need > s->table_cap) {
+            std::lock_guard<std::mutex> g(s->m);
+            s->last_error = "ByteCRT server: the plan tables need " + std::to_string(need) +
+                            " bytes of HBM, above the budget's table share of " + std::to_string(s->table_cap);
+            return BCRT_EINVAL;
+        }
+        for (;;) {
+            if (s->table_used + need <= s->table_cap) break;
+            Plan* lru = nullptr;
+            std::string lru_key;
+            for (auto& kv : s->plans)
+                if (kv.second->refs == 0 && (!lru || kv.second->last_use < lru->last_use)) {
+                    lru = kv.second.get();
+                    lru_key = kv.first;
+                }
+            if (lru) {
+                s->table_used -= lru->table_bytes;
+                s->plans.erase(lru_key);  // frees the tables (no request uses them)
+                ++s->evictions;
+                continue;
+            }
+            s->plan_cv.wait(pg);  // every resident plan is in use: wait for one to go idle
+            it = s->plans.find(key);  // another submitter may have built it meanwhile
+            if (it != s->plans.end()) break;
+        }
+        if (it == s->plans.end()) {
+            auto p = std::make_unique<Plan>();
+            char msg[256] = {0};
+            const int rc = bcrt::create_tables_only(&p->engine, N, W, L, q, msg, sizeof(msg));
+            if (rc != BCRT_OK) {
+                std::lock_guard<std::mutex> g(s->m);
+                s->last_error = msg;
+                return rc;
+            }
+            p->t = bcrt::engine_tables(p->engine);
+            p->io_words = size_t(N) * L;
+            p->plane_words = size_t(2) * p->t.P * N;
+            p->table_bytes = bcrt::engine_table_bytes(p->engine);
+            s->table_used += p->table_bytes;
+            it = s->plans.emplace(key, std::move(p)).first;
+        }
+    }
+    Plan* plan = it->second.get();
+    ++plan->refs;
+    plan->last_use = ++s->plan_tick;
+    *out = plan;
+    return BCRT_OK;
 }
 
 extern "C" {
@@ -276,7 +371,10 @@ int bcrt_server_create(bcrt_server** out, size_t hbm_budget_bytes, uint32_t max_
     try {
         if (hbm_budget_bytes < (size_t(1) << 20) || max_wave == 0)
             throw std::invalid_argument("ByteCRT server needs a budget >= 1 MiB and max_wave >= 1");
-        s->cap[kArena] = hbm_budget_bytes / kAlign * kAlign;
+        // Three quarters of the budget are the request arena, one quarter holds
+        // resident plan tables.
+        s->table_cap = hbm_budget_bytes / 4;
+        s->cap[kArena] = (hbm_budget_bytes - s->table_cap) / kAlign * kAlign;
         // Operands are at most half and results a quarter of a product's
         // device footprint (its residue planes outweigh the operands).
         s->cap[kPinIn] = s->cap[kArena] / 2 / kAlign * kAlign;
@@ -312,41 +410,37 @@ int bcrt_server_create(bcrt_server** out, size_t hbm_budget_bytes, uint32_t max_
 int bcrt_server_submit(bcrt_server* s, uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, const uint32_t* a,
                        const uint32_t* b, uint32_t* c, uint64_t* ticket) {
     if (!s || !q || !a || !b || !c || !ticket) return BCRT_EINVAL;
-    Plan* plan = nullptr;
-    {
-        std::lock_guard<std::mutex> pg(s->plan_m);
-        std::string key(reinterpret_cast<const char*>(q), size_t(L) * 4);
-        key += "|" + std::to_string(N) + "|" + std::to_string(W);
-        auto it = s->plans.find(key);
-        if (it == s->plans.end()) {
-            auto p = std::make_unique<Plan>();
-            char msg[256] = {0};
-            const int rc = bcrt_create(&p->engine, 1, N, W, L, q, msg, sizeof(msg));
-            if (rc != BCRT_OK) {
-                std::lock_guard<std::mutex> g(s->m);
-                s->last_error = msg;
-                return rc;
-            }
-            p->t = bcrt::engine_tables(p->engine);
-            p->io_words = size_t(N) * L;
-            p->plane_words = size_t(2) * p->t.P * N;
-            it = s->plans.emplace(key, std::move(p)).first;
-        }
-        plan = it->second.get();
+    // Geometry and the request's footprint first: nothing is built for a
+    // request that could never be admitted.
+    uint32_t P = 0;
+    try {
+        if (N < 4096 || N > 65536 || (N & (N - 1)) || W < 64 || L != (W + 31) / 32 || L > 64)
+            throw std::invalid_argument("ByteCRT plan: need N = 2^k in [4096, 65536], 64 <= W <= 2048, L = ceil(W/32)");
+        P = bcrt::plan_prime_count(N, W);
+    } catch (const std::exception& x) {
+        std::lock_guard<std::mutex> g(s->m);
+        s->last_error = x.what();
+        return dynamic_cast<const std::invalid_argument*>(&x) ? BCRT_EINVAL : BCRT_ERUNTIME;
     }
     bcrt::SchedItem item;
+    for (int k = 0; k < bcrt::kRings; ++k) item.need[k] = footprint(size_t(N) * L, size_t(2) * P * N, k);
+    {
+        std::lock_guard<std::mutex> g(s->m);
+        if (!s->sched->fits_alone(item)) {
+            s->last_error = "ByteCRT server: one product needs " + std::to_string(item.need[kArena]) +
+                            " bytes of HBM, above the budget";
+            return BCRT_EINVAL;
+        }
+        if (!s->fatal.empty()) {
+            s->last_error = s->fatal;
+            return BCRT_ERUNTIME;
+        }
+    }
+    Plan* plan = nullptr;
+    const int rc = acquire_plan(s, N, W, L, q, P, &plan);
+    if (rc != BCRT_OK) return rc;
     item.plan = plan;
-    for (int k = 0; k < bcrt::kRings; ++k) item.need[k] = footprint(plan, k);
     std::lock_guard<std::mutex> g(s->m);
-    if (!s->sched->fits_alone(item)) {
-        s->last_error = "ByteCRT server: one product needs " + std::to_string(item.need[kArena]) +
-                        " bytes of HBM, above the budget";
-        return BCRT_EINVAL;
-    }
-    if (!s->fatal.empty()) {
-        s->last_error = s->fatal;
-        return BCRT_ERUNTIME;
-    }
     auto r = std::make_unique<Request>();
     r->id = s->next_id++;
     r->plan = plan;
@@ -381,16 +475,24 @@ int bcrt_server_wait(bcrt_server* s, uint64_t ticket, double* latency_s) {
 
 int bcrt_server_stats(bcrt_server* s, uint64_t* out) {
     if (!s || !out) return BCRT_EINVAL;
-    size_t nplans;
+    size_t nplans, table_used, table_cap;
+    uint64_t evictions;
     {
         std::lock_guard<std::mutex> pg(s->plan_m);
         nplans = s->plans.size();
+        table_used = s->table_used;
+        table_cap = s->table_cap;
+        evictions = s->evictions;
     }
     std::lock_guard<std::mutex> g(s->m);
     out[0] = s->completed;
     out[1] = s->waves;
     out[2] = s->max_wave_seen;
     out[3] = nplans;
+    out[4] = evictions;
+    out[5] = table_used;
+    out[6] = table_cap;
+    out[7] = s->cap[kArena];
     return BCRT_OK;
 }
 

@@ -117,6 +117,8 @@ typedef struct bcrt_group {
 
 typedef struct bcrt_mix bcrt_mix;
 
+/* The mix holds its engines' tables and workspace: bcrt_destroy of an engine
+ * that a live mix names is deferred until the last such mix is destroyed. */
 int bcrt_mix_create(bcrt_mix** out, const bcrt_group* groups, uint32_t ngroups, char* err, size_t errlen);
 /* Enqueue the mixed batch on `stream` (hipStream_t, NULL = default); graph
  * != 0 replays a hipGraph captured on first use. */
@@ -126,12 +128,19 @@ void bcrt_mix_destroy(bcrt_mix* m);
 
 /* ---- Serving: continuous batching of single-product requests ----
  * No reference counterpart (the reference serves one fixed-size batch per
- * fherma_run call).  Requests of any geometry are admitted, in arrival
- * order, into waves while the device arena (hbm_budget_bytes, holding each
- * request's operands, result and residue planes) and pinned staging have
- * room; each wave runs as one mixed launch per stage on one of two compute
- * streams, so waves admitted while others are on the GPU overlap with them.
- * Plan tables stay resident per (N, W, q) for the server's lifetime.
+ * fherma_run call; its per-init state is built once, native-batched/solve.cu:160-198).
+ * hbm_budget_bytes bounds the server's device memory: three quarters are the
+ * request arena (each request's operands, result and residue planes), one
+ * quarter holds resident plan tables (twiddles and MFMA weights per (N, W, q),
+ * no workspace).  Requests of any geometry are admitted, in arrival order,
+ * into waves while the arena and pinned staging have room; each wave runs as
+ * one mixed launch per stage on one of two compute streams, so waves admitted
+ * while others are on the GPU overlap with them.  A request whose plan is not
+ * resident builds it, first evicting idle plans (no request pending or in
+ * flight) least recently used first until its tables fit; submit blocks
+ * while every resident plan that could make room is still in use.  A request
+ * larger than the arena, or a plan larger than the table share, is refused
+ * with BCRT_EINVAL before anything is built.
  * a, b, c are host [N][L] buffers that must stay valid until the request is
  * waited on. */
 typedef struct bcrt_server bcrt_server;
@@ -140,7 +149,8 @@ int bcrt_server_submit(bcrt_server* s, uint32_t N, uint32_t W, uint32_t L, const
                        const uint32_t* b, uint32_t* c, uint64_t* ticket);
 /* Blocks until the request is complete; latency_s = submit-to-completion. */
 int bcrt_server_wait(bcrt_server* s, uint64_t ticket, double* latency_s);
-/* out[0..3] = completed requests, waves launched, largest wave, resident plans. */
+/* out[0..7] = completed requests, waves launched, largest wave, resident
+ * plans, plan evictions, table bytes resident, table share, arena bytes. */
 int bcrt_server_stats(bcrt_server* s, uint64_t* out);
 const char* bcrt_server_last_error(const bcrt_server* s);
 void bcrt_server_destroy(bcrt_server* s);
