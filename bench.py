@@ -401,14 +401,18 @@ def sweep_mode(args, torch):
     fus = tune.tune_fusion(eng, a, b, c, B, reps=args.reps, stream=stream)
     for r in fus["rows"]:
         print(json.dumps(r), flush=True)
-    res = tune.tune_pipeline(eng, a, b, c, B, reps=args.reps, stream=stream)
-    for r in res["rows"]:
+    try:
+        res = tune.tune_pipeline(eng, a, b, c, B, reps=args.reps, stream=stream)
+    except RuntimeError as x:  # the pipeline tuner is switched off
+        print(json.dumps({"pipeline": str(x)}), flush=True)
+        res = None
+    for r in (res or {}).get("rows", []):
         print(json.dumps(r), flush=True)
     out = Path(args.sweep_out)
     out.parent.mkdir(parents=True, exist_ok=True)
     tune.record(N, W, B, res, device=torch.cuda.get_device_name(0), path=out, encode=enc, fusion=fus)
-    print(json.dumps({"best": res["best"], "encode_tile": enc["best"], "fusion": fus["best"], "table": str(out)}),
-          flush=True)
+    print(json.dumps({"best": res and res["best"], "encode_tile": enc["best"], "fusion": fus["best"],
+                      "table": str(out)}), flush=True)
 
 
 def main():
