@@ -21,7 +21,7 @@ LIB = PKG / "lib"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["bytecrt_kernels.hip", "bytecrt_plan.cpp", "bytecrt_io.cpp", "bytecrt_trace.cpp",
+SOURCES = ["bytecrt_kernels.hip", "bytecrt_noalloc.hip", "bytecrt_plan.cpp", "bytecrt_io.cpp", "bytecrt_trace.cpp",
            "bytecrt_engine.cpp", "bytecrt_server.cpp", "fherma_solve.cpp"]
 # harness_main.cpp is linked separately into lib/bytecrt_solution.
 

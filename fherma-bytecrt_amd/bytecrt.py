@@ -33,7 +33,7 @@ EXPORTED = (
     "bcrt_set_encode_tile", "bcrt_set_fusion",
     "bcrt_mix_create", "bcrt_mix_run", "bcrt_mix_last_error", "bcrt_mix_destroy",
     "bcrt_server_create", "bcrt_server_submit", "bcrt_server_wait", "bcrt_server_stats",
-    "bcrt_server_last_error", "bcrt_server_destroy",
+    "bcrt_server_last_error", "bcrt_server_destroy", "bcrt_run_device_noalloc",
 )
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -62,7 +62,7 @@ def load_library():
     lib.bcrt_destroy.argtypes = [vp]
     lib.bcrt_run.restype = ctypes.c_int
     lib.bcrt_run.argtypes = [vp, _u32p, _u32p, _u32p, ctypes.c_uint32]
-    for name in ("bcrt_run_device", "bcrt_run_device_graph"):
+    for name in ("bcrt_run_device", "bcrt_run_device_graph", "bcrt_run_device_noalloc"):
         f = getattr(lib, name)
         f.restype = ctypes.c_int
         f.argtypes = [vp, vp, vp, vp, ctypes.c_uint32, vp]
@@ -209,6 +209,12 @@ class Engine:
         """Enqueue on device tensors (no sync).  graph=True replays a captured hipGraph."""
         f = load_library().bcrt_run_device_graph if graph else load_library().bcrt_run_device
         self._check(f(self._h, _dptr(a), _dptr(b), _dptr(c), batch, _stream_handle(stream)))
+
+    def run_device_noalloc(self, a, b, c, batch: int, stream=None):
+        """Enqueue on device tensors with the residue planes in the library's
+        fixed 64 MiB device scratch array: nothing is allocated (no sync)."""
+        self._check(load_library().bcrt_run_device_noalloc(self._h, _dptr(a), _dptr(b), _dptr(c), batch,
+                                                           _stream_handle(stream)))
 
     def run_stages(self, mask: int, a=None, b=None, c=None, planes=None, batch: int = 1, stream=None):
         lib = load_library()

@@ -755,3 +755,21 @@ extern "C" void bcrt_mix_destroy(bcrt_mix* m) {
 namespace bcrt {
 const DevTables& engine_tables(const bcrt_engine* e) { return e->tables; }
 }  // namespace bcrt
+
+namespace bcrt {
+
+int engine_device(const bcrt_engine* e) { return e->device; }
+
+int engine_fail(bcrt_engine* e, int code, const std::string& msg) { return fail(e, code, msg); }
+
+// The product for a batch with caller-given residue planes (2 * P * N * batch
+// words), on one stream: no pipelining, no workspace, nothing allocated.
+int run_with_planes(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t* planes,
+                    uint32_t batch, void* stream) {
+    return guarded(e, [&] {
+        const bcrt::BatchArgs args{d_a, d_b, d_c, planes, batch};
+        check(bcrt::launch_product(e->tables, args, pick(e, stream)), "launch product");
+    });
+}
+
+}  // namespace bcrt

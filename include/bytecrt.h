@@ -74,6 +74,19 @@ int bcrt_run_device(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, ui
 int bcrt_run_device_graph(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
                           uint32_t batch, void* stream);
 
+/* Allocation-free variant of bcrt_run_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * residue planes (2 * P * N * batch words) live in a 64 MiB device array that
+ * the library holds on each GPU from its first call there until the process
+ * ends, instead of the engine's workspace; the call allocates nothing, runs
+ * on `stream` without pipelining and ignores max_batch.  A batch whose planes
+ * exceed 64 MiB is refused (BCRT_EINVAL): one product fits at every geometry,
+ * four at N = 32,768, W = 868.  Calls that overlap on one GPU share the array
+ * and race. */
+int bcrt_run_device_noalloc(bcrt_engine* e, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
+                            uint32_t batch, void* stream);
+
 /* Batch pipelining (no reference counterpart; the reference overlapped four
  * product slots on three streams, native-batched/solve.cu:17-18,56-156):
  * batches larger than `chunk` products are split into chunks that run on
