@@ -40,8 +40,10 @@ def parse():
     ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
-    ap.add_argument("--only-stage", choices=["encode", "fwd_cols", "encode_fwd", "middle", "inv_cols", "crt", "product"],
-                    help="profiling mode: run one stage (or the whole product) --reps times on the batch")
+    ap.add_argument("--only-stage", choices=["encode", "fwd_cols", "encode_fwd", "middle", "inv_cols", "crt", "product",
+                                             "hbm_copy", "hbm_read", "hbm_write", "bfly_fp64", "bfly_shoup32",
+                                             "bfly_montgomery32"],
+                    help="profiling mode: run one stage (or the whole product, or one probe) --reps times")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--sweep", action="store_true",
                     help="autotune the batch pipelining, write the pipeline table and exit")
@@ -55,21 +57,40 @@ def parse():
 STAGE_MASK = {"encode": 1, "fwd_cols": 2, "encode_fwd": 3, "middle": 4, "inv_cols": 8, "crt": 16, "product": 31}
 
 
+def probe_tw15(p: int, psi: int):
+    """15 twiddles for the butterfly probes: psi^1 .. psi^15 mod p."""
+    return [pow(psi, k, p) for k in range(1, 16)]
+
+
 def profile_mode(args, torch):
     """Work for rocprofv3 runs: nothing but the named kernels after setup."""
-    if args.copy_probe_mib:
-        n = args.copy_probe_mib * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.int32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(args.reps):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": n * 4, "reps": args.reps}), flush=True)
-        return
     sys.path.insert(0, str(ROOT))
     import __graft_entry__ as ge
     pkg = ge.load_package()
+    probes = pkg.probes
+    if args.copy_probe_mib or args.only_stage.startswith("hbm_"):
+        nbytes = (args.copy_probe_mib or 2048) * (1 << 20)
+        mode = 0 if args.copy_probe_mib else {"hbm_copy": 0, "hbm_read": 1, "hbm_write": 2}[args.only_stage]
+        src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(args.reps):
+            probes.hbm_stream(mode, src, dst, nbytes)
+        torch.cuda.synchronize()
+        print(json.dumps({"hbm_probe": probes.MODES[mode], "bytes": nbytes, "reps": args.reps}), flush=True)
+        return
     q = pkg.datagen.modulus(W, 2026)
+    if args.only_stage.startswith("bfly_"):
+        pl = pkg.bytecrt.plan_host(N, W, q)
+        p, psi = int(pl["primes"][0]), int(pl["psi"][0])
+        form = {"bfly_fp64": 0, "bfly_shoup32": 1, "bfly_montgomery32": 2}[args.only_stage]
+        threads, rounds = 1 << 20, 64
+        out = torch.empty(threads * 16, dtype=torch.int32, device="cuda")
+        for _ in range(args.reps):
+            probes.butterflies(form, out, threads, p, probe_tw15(p, psi), rounds)
+        torch.cuda.synchronize()
+        print(json.dumps({"butterfly_probe": probes.FORMS[form], "butterflies": threads * rounds * 32,
+                          "reps": args.reps}), flush=True)
+        return
     eng = pkg.Engine(N, W, q, max_batch=B)
     a = device_operands(torch, B, N, q, 11)
     b = device_operands(torch, B, N, q, 12)
@@ -350,32 +371,59 @@ def serving_leg(pkg, qps: float, seconds: float, budget_gib: float):
             "waves": st["waves"] - len(SERVE_GEOMS), "max_wave": st["max_wave"], "exact": all(ok)}
 
 
-def cpu_baseline(pkg, q, seconds: float):
+def cpu_threads():
+    """Threads for the CPU baseline: the CPU share this process is given
+    (OMP_NUM_THREADS where the host sets it, which is how a GPU host states
+    one GPU's share of its cores), else every CPU of the affinity mask."""
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    share = int(env) if env.isdigit() and int(env) > 0 else affinity
+    return min(share, affinity), affinity
+
+
+def cpu_model():
+    cpu = platform.processor() or "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return cpu
+
+
+def cpu_products(pkg, n: int, w: int, q: int, seconds: float, threads: int):
+    """The oracle's full product (C restatement of the reference algorithm,
+    OpenMP) on seeded operands of one geometry, repeated for about `seconds`."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = min(threads, 16)
     oracle.set_threads(threads)
-    pl = oracle.Plan(N, W, q)
-    a = pkg.datagen.operands(N, q, 1001)
-    b = pkg.datagen.operands(N, q, 1002)
+    pl = oracle.Plan(n, w, q)
+    a = pkg.datagen.operands(n, q, 1001)
+    b = pkg.datagen.operands(n, q, 1002)
     done, t0 = 0, time.perf_counter()
     while True:
         pl.product(a, b)
         done += 1
         el = time.perf_counter() - t0
-        if el >= seconds or done >= 64:
+        if el >= seconds or done >= 256:
             break
-    cpu = platform.processor() or "unknown"
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
-    return {"value": done / el, "unit": "products/s", "cores": threads, "kind": "port",
-            "sample": f"{done} full products N={N} W={W} (oracle C restatement, OpenMP) in {el:.1f} s on {cpu}"}
+    return done, el
+
+
+def cpu_baseline(pkg, q, seconds: float):
+    threads, affinity = cpu_threads()
+    done, el = cpu_products(pkg, N, W, q, seconds, threads)
+    n1, w1 = 32768, 868
+    q1 = pkg.datagen.modulus(w1, 2026)
+    done1, el1 = cpu_products(pkg, n1, w1, q1, max(seconds / 3, 2.0), threads)
+    cpu = cpu_model()
+    return {"value": round(done / el, 3), "unit": "products/s", "cores": threads, "kind": "port",
+            "affinity_cpus": affinity,
+            "sample": f"{done} full products N={N} W={W} (oracle C restatement of the reference algorithm, "
+                      f"OpenMP, {threads} threads) in {el:.1f} s on {cpu}",
+            "config1": {"N": n1, "W": w1, "value": round(done1 / el1, 3), "unit": "products/s",
+                        "sample": f"{done1} full products in {el1:.1f} s, same threads"}}
 
 
 def sweep_mode(args, torch):
@@ -477,8 +525,15 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
-    batch_ms = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
     value = B / (ms_per_step / 1e3)
+    # -- batched call latency: >= 100 graph replays, each waited for ----------
+    lat_b = []
+    for _ in range(max(100, args.steps)):
+        t1 = time.perf_counter()
+        eng.run_device(a, b, c, B, stream=stream, graph=True)
+        stream.synchronize()
+        lat_b.append(time.perf_counter() - t1)
+    batch_ms = np.array(lat_b) * 1e3
 
     # -- per-kernel durations on the same stream (hip events) ----------------
     # The product path's kernels (encode and forward columns fused), then the
@@ -515,6 +570,19 @@ def main():
         roof = {"bound": "hbm", "achieved": stages[dom]["hbm_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": stages[dom]["hbm_frac"], "traffic": measured_traffic(dom), "kernel": dom,
                 "algorithmic_bytes_per_launch": sb[dom] * B}
+    # Measured ceilings: hand-written HBM stream kernels and the butterfly
+    # rate of each exact modular-multiplication form (probes library).
+    pt = eng.plan_tables()
+    p0, psi0 = int(pt["primes"][0]), int(pt["psi"][0])
+    ceilings = {"hbm": pkg.probes.measure_hbm(torch),
+                "butterflies": pkg.probes.measure_butterflies(torch, p0, probe_tw15(p0, psi0))}
+    if roof["bound"] == "hbm":
+        roof["practical_peak"] = ceilings["hbm"]["copy_GBps"]
+        roof["practical_frac"] = round(roof["achieved"] / roof["practical_peak"], 4)
+    # NTT butterflies per batch (2 forward + 1 inverse transform per prime,
+    # N/2 log2 N each) at the measured FP64 butterfly rate
+    ntt_bfly = 3 * P * (N // 2) * 16 * B
+    ceilings["ntt_fp64_floor_ms"] = round(ntt_bfly / (ceilings["butterflies"]["fp64_Tbfly_per_s"] * 1e12) * 1e3, 3)
     # NTT-only kernels of the product path (the forward column levels run
     # inside the fused encode kernel)
     ntt_ms = dur["middle"] + dur["inv_cols"]
@@ -544,7 +612,7 @@ def main():
         "plain": {"products_per_s": round(plain_pps, 2), "calls": args.plain_calls,
                   "p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
                   "p99_ms": round(float(np.percentile(lat_ms, 99)), 4)},
-        "batched": {"products_per_s": round(value, 2),
+        "batched": {"products_per_s": round(value, 2), "replays": int(batch_ms.size),
                     "p50_ms": round(float(np.percentile(batch_ms, 50)), 4),
                     "p99_ms": round(float(np.percentile(batch_ms, 99)), 4)},
         "batched_vs_plain": round(value / plain_pps, 3),
@@ -553,7 +621,9 @@ def main():
         "stages": stages,
         "unfused_stages": unfused,
         "ntt_hbm_frac": round(ntt_gbs / HBM_PEAK_GBS, 4),
+        "ntt_practical_frac": round(ntt_gbs / ceilings["hbm"]["copy_GBps"], 4),
         "roofline": roof,
+        "ceilings": ceilings,
         "whole_call": whole,
         "uneven": uneven,
         "serving": serving,

@@ -4,6 +4,7 @@
 
 Outputs (git-ignored, shipped in-tree to the GPU host):
     fherma-bytecrt_amd/lib/libbytecrt.so      C ABI (include/bytecrt.h) + fherma.h wrapper
+    fherma-bytecrt_amd/lib/libbytecrt_probes.so  measurement probes (include/bytecrt_probes.h)
     fherma-bytecrt_amd/lib/bytecrt_solution   point-directory runner (reference wire format)
 """
 from __future__ import annotations
@@ -64,6 +65,10 @@ def build(verbose: bool = False) -> Path:
     os.replace(tmp, out)
     for o in objs:
         os.remove(o)
+    # Measurement probes (include/bytecrt_probes.h): a library of their own.
+    probes = LIB / "libbytecrt_probes.so"
+    _run([HIPCC, *DEVICE, *COMMON, "-shared", str(CSRC / "bytecrt_probes.hip"), "-o", str(LIB / "libbytecrt_probes.so.tmp")])
+    os.replace(LIB / "libbytecrt_probes.so.tmp", probes)
     # The harness runner (reference wire format) links the library by rpath.
     # So does the whole-call timer (fherma_run at the harness boundary).
     for exe, src in (("bytecrt_solution", "harness_main.cpp"), ("fherma_call_bench", "fherma_call_bench.cpp")):

@@ -1,0 +1,207 @@
+// Measurement probes for gfx950 (include/bytecrt_probes.h): HBM stream
+// kernels and butterfly kernels for three exact modular multiplications.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/bytecrt.h"
+#include "../../include/bytecrt_probes.h"
+
+namespace {
+
+constexpr uint32_t kThreads = 256;
+constexpr int kInFlight = 4;       // 16-byte accesses per thread in flight
+constexpr uint32_t kGridPerCu = 32;  // workgroups per CU (a sweep of 4..32 on MI355X: 32 is best)
+
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
+// Grid-stride over 16-byte words; each step issues kInFlight loads before its
+// stores (copy), or only loads (read), or only stores (write).  Copy and read
+// use non-temporal accesses (the data is touched once).
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_stream(const v4u* __restrict__ src, v4u* __restrict__ dst, size_t n) {
+    const size_t stride = size_t(gridDim.x) * kThreads;
+    const size_t tid = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    v4u acc = v4u{0, 0, 0, 0};
+    size_t i = tid;
+    for (; i + (kInFlight - 1) * stride < n; i += kInFlight * stride) {
+        if constexpr (MODE == 2) {
+#pragma unroll
+            for (int k = 0; k < kInFlight; ++k) dst[i + k * stride] = v4u{uint32_t(i), 1u, 2u, uint32_t(k)};
+        } else {
+            v4u v[kInFlight];
+#pragma unroll
+            for (int k = 0; k < kInFlight; ++k) v[k] = __builtin_nontemporal_load(src + i + k * stride);
+#pragma unroll
+            for (int k = 0; k < kInFlight; ++k) {
+                if constexpr (MODE == 0)
+                    __builtin_nontemporal_store(v[k], dst + i + k * stride);
+                else
+                    acc ^= v[k];
+            }
+        }
+    }
+    for (; i < n; i += stride) {
+        if constexpr (MODE == 0) dst[i] = src[i];
+        if constexpr (MODE == 1) acc ^= src[i];
+        if constexpr (MODE == 2) dst[i] = v4u{uint32_t(i), 1u, 2u, 3u};
+    }
+    if constexpr (MODE == 1) dst[tid] = acc;
+}
+
+// ---- butterfly forms -------------------------------------------------------
+struct Twiddles {
+    uint32_t w[15];      // w in [0, p)
+    uint32_t shoup[15];  // floor(w 2^32 / p)
+    uint32_t mont[15];   // w 2^32 mod p
+    uint32_t p;
+    uint32_t pneg_inv;   // -p^-1 mod 2^32
+    double pd, pinv;
+};
+
+constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
+
+// The product kernels' FP64 form: exact for |x| <= 2^40, |r| <= p/2 + 2^-12 p.
+__device__ __forceinline__ double mul_mod_f64(double x, double w, double p, double pinv) {
+    const double h = x * w;
+    const double l = fma(x, w, -h);
+    const double q = fma(h, pinv, kRound) - kRound;
+    return fma(-q, p, h) + l;
+}
+
+// x in [0, p), w in [0, p), ws = floor(w 2^32 / p): x w mod p in [0, p).
+__device__ __forceinline__ uint32_t mul_mod_shoup(uint32_t x, uint32_t w, uint32_t ws, uint32_t p) {
+    const uint32_t q = __umulhi(x, ws);
+    const uint32_t r = x * w - q * p;  // in [0, 2p), exact mod 2^32
+    return min(r, r - p);
+}
+
+// x in [0, p), wm = w 2^32 mod p: REDC(x wm) = x w mod p in [0, p).
+__device__ __forceinline__ uint32_t mul_mod_mont(uint32_t x, uint32_t wm, uint32_t p, uint32_t pneg_inv) {
+    const uint64_t t = uint64_t(x) * wm;
+    const uint32_t m = uint32_t(t) * pneg_inv;
+    const uint64_t u = (t + uint64_t(m) * p) >> 32;  // < 2p
+    const uint32_t r = uint32_t(u);
+    return min(r, r - p);
+}
+
+__device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b, uint32_t p) {
+    const uint32_t s = a + b;  // < 2^32 for p < 2^31
+    return min(s, s - p);
+}
+
+__device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) {
+    const uint32_t d = a - b;
+    return min(d, d + p);
+}
+
+// 16 starting residues of a thread, canonical.
+__device__ __forceinline__ uint32_t start_value(size_t tid, int k, uint32_t p) {
+    uint64_t z = uint64_t(tid) * 16 + uint64_t(k) + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return uint32_t((z ^ (z >> 31)) % p);
+}
+
+template <int FORM>
+__global__ __launch_bounds__(kThreads) void k_bfly(uint32_t* __restrict__ out, const Twiddles tw, uint32_t rounds) {
+    const size_t tid = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    const uint32_t p = tw.p;
+    if constexpr (FORM == 0) {
+        double v[16], w[15];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) w[k] = double(tw.w[k]);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = double(start_value(tid, k, p));
+        for (uint32_t r = 0; r < rounds; ++r) {
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const int half = 8 >> l;
+#pragma unroll
+                for (int g = 0; g < (1 << l); ++g)
+#pragma unroll
+                    for (int x = 0; x < half; ++x) {
+                        const int i0 = g * 2 * half + x;
+                        const double t = mul_mod_f64(v[i0 + half], w[(1 << l) - 1 + g], tw.pd, tw.pinv);
+                        v[i0 + half] = v[i0] - t;
+                        v[i0] = v[i0] + t;
+                    }
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = fma(-(fma(v[k], tw.pinv, kRound) - kRound), tw.pd, v[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const double r = v[k] < 0.0 ? v[k] + tw.pd : v[k];
+            out[tid * 16 + k] = uint32_t(r);
+        }
+    } else {
+        uint32_t v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = start_value(tid, k, p);
+        for (uint32_t r = 0; r < rounds; ++r) {
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const int half = 8 >> l;
+#pragma unroll
+                for (int g = 0; g < (1 << l); ++g)
+#pragma unroll
+                    for (int x = 0; x < half; ++x) {
+                        const int i0 = g * 2 * half + x, k = (1 << l) - 1 + g;
+                        const uint32_t t = FORM == 1 ? mul_mod_shoup(v[i0 + half], tw.w[k], tw.shoup[k], p)
+                                                     : mul_mod_mont(v[i0 + half], tw.mont[k], p, tw.pneg_inv);
+                        v[i0 + half] = sub_mod(v[i0], t, p);
+                        v[i0] = add_mod(v[i0], t, p);
+                    }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) out[tid * 16 + k] = v[k];
+    }
+}
+
+}  // namespace
+
+extern "C" int bcrtp_hbm_stream(int mode, const void* d_src, void* d_dst, size_t bytes, void* stream) {
+    if (mode < 0 || mode > 2 || bytes == 0 || bytes % 16 || !d_dst || (mode != 2 && !d_src)) return BCRT_EINVAL;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return BCRT_ERUNTIME;
+    const size_t n = bytes / 16;
+    const dim3 grid(uint32_t(cus) * kGridPerCu), blk(kThreads);
+    if (mode == 1 && n < size_t(grid.x) * kThreads) return BCRT_EINVAL;  // d_dst holds one word per thread
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const v4u* src = static_cast<const v4u*>(d_src);
+    v4u* dst = static_cast<v4u*>(d_dst);
+    if (mode == 0) hipLaunchKernelGGL(k_stream<0>, grid, blk, 0, s, src, dst, n);
+    if (mode == 1) hipLaunchKernelGGL(k_stream<1>, grid, blk, 0, s, src, dst, n);
+    if (mode == 2) hipLaunchKernelGGL(k_stream<2>, grid, blk, 0, s, src, dst, n);
+    return hipGetLastError() == hipSuccess ? BCRT_OK : BCRT_ERUNTIME;
+}
+
+extern "C" int bcrtp_butterflies(int form, uint32_t* d_out, size_t threads, uint32_t p, const uint32_t* tw15,
+                                 uint32_t rounds, void* stream) {
+    if (form < 0 || form > 2 || !d_out || !tw15 || threads == 0 || threads % kThreads || p <= (1u << 30) ||
+        p >= (1u << 31) || (p & 1) == 0 || threads / kThreads > 0x7FFFFFFF)
+        return BCRT_EINVAL;
+    Twiddles tw{};
+    tw.p = p;
+    uint32_t inv = 1;  // p^-1 mod 2^32 by Newton iteration (p odd)
+    for (int i = 0; i < 5; ++i) inv *= 2 - p * inv;
+    tw.pneg_inv = 0u - inv;
+    tw.pd = double(p);
+    tw.pinv = 1.0 / double(p);
+    for (int k = 0; k < 15; ++k) {
+        if (tw15[k] >= p) return BCRT_EINVAL;
+        tw.w[k] = tw15[k];
+        tw.shoup[k] = uint32_t((uint64_t(tw15[k]) << 32) / p);
+        tw.mont[k] = uint32_t((uint64_t(tw15[k]) << 32) % p);
+    }
+    const dim3 grid(uint32_t(threads / kThreads)), blk(kThreads);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (form == 0) hipLaunchKernelGGL(k_bfly<0>, grid, blk, 0, s, d_out, tw, rounds);
+    if (form == 1) hipLaunchKernelGGL(k_bfly<1>, grid, blk, 0, s, d_out, tw, rounds);
+    if (form == 2) hipLaunchKernelGGL(k_bfly<2>, grid, blk, 0, s, d_out, tw, rounds);
+    return hipGetLastError() == hipSuccess ? BCRT_OK : BCRT_ERUNTIME;
+}

@@ -1,0 +1,101 @@
+"""Measurement probes of include/bytecrt_probes.h over ctypes.
+
+``measure_hbm`` times hand-written HBM stream kernels (copy, read, write) --
+the practical ceiling the product's kernels are compared with, besides the
+published 8 TB/s.  ``measure_butterflies`` times the same radix-16 butterfly
+rounds in the three exact modular-multiplication forms a 31-bit-prime NTT can
+use on gfx950 (FP64 as the product kernels use it, 32-bit Shoup, 32-bit
+Montgomery) and checks that all three produce the same residues.  Timing uses
+HIP events on one torch stream; torch only holds device memory.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libbytecrt_probes.so"
+EXPORTED = ("bcrtp_hbm_stream", "bcrtp_butterflies")
+FORMS = {0: "fp64", 1: "shoup32", 2: "montgomery32"}
+MODES = {0: "copy", 1: "read", 2: "write"}
+_LIB = None
+
+
+def load_library():
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"ByteCRT probe library missing: {LIB_PATH} (run __graft_entry__.build())")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        vp = ctypes.c_void_p
+        lib.bcrtp_hbm_stream.restype = ctypes.c_int
+        lib.bcrtp_hbm_stream.argtypes = [ctypes.c_int, vp, vp, ctypes.c_size_t, vp]
+        lib.bcrtp_butterflies.restype = ctypes.c_int
+        lib.bcrtp_butterflies.argtypes = [ctypes.c_int, vp, ctypes.c_size_t, ctypes.c_uint32,
+                                          ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, vp]
+        _LIB = lib
+    return _LIB
+
+
+def hbm_stream(mode: int, src, dst, nbytes: int, stream=None):
+    rc = load_library().bcrtp_hbm_stream(mode, src.data_ptr() if src is not None else None, dst.data_ptr(),
+                                        nbytes, stream.cuda_stream if stream is not None else None)
+    if rc != 0:
+        raise RuntimeError(f"ByteCRT HBM probe mode {mode} failed ({rc})")
+
+
+def butterflies(form: int, out, threads: int, p: int, tw15, rounds: int, stream=None):
+    arr = (ctypes.c_uint32 * 15)(*[int(w) for w in tw15])
+    rc = load_library().bcrtp_butterflies(form, out.data_ptr(), threads, p, arr, rounds,
+                                         stream.cuda_stream if stream is not None else None)
+    if rc != 0:
+        raise RuntimeError(f"ByteCRT butterfly probe form {form} failed ({rc})")
+
+
+def _timed(torch, stream, fn, reps: int) -> float:
+    """Median milliseconds of fn() over reps launches (HIP events on stream)."""
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fn()
+        b.record(stream)
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def measure_hbm(torch, nbytes: int = 2 << 30, reps: int = 5) -> dict:
+    """GB/s of the copy (read + write bytes), read and write stream kernels."""
+    src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.Stream()
+    out = {"bytes": nbytes}
+    for mode, name in MODES.items():
+        ms = _timed(torch, stream, lambda: hbm_stream(mode, src, dst, nbytes, stream), reps)
+        moved = nbytes * (2 if mode == 0 else 1)
+        out[f"{name}_GBps"] = round(moved / (ms * 1e-3) / 1e9, 1)
+    hbm_stream(0, src, dst, nbytes, stream)  # the write probe ran last: copy again and check it
+    torch.cuda.synchronize()
+    out["copy_exact"] = bool(torch.equal(dst, src))
+    del src, dst
+    return out
+
+
+def measure_butterflies(torch, p: int, tw15, threads: int = 1 << 20, rounds: int = 64, reps: int = 5) -> dict:
+    """Butterflies per second (x 1e12) of each form, and whether all forms
+    produced the same residues."""
+    stream = torch.cuda.Stream()
+    outs, res = {}, {"threads": threads, "rounds": rounds, "p": p}
+    for form, name in FORMS.items():
+        out = torch.empty(threads * 16, dtype=torch.int32, device="cuda")
+        ms = _timed(torch, stream, lambda: butterflies(form, out, threads, p, tw15, rounds, stream), reps)
+        res[f"{name}_Tbfly_per_s"] = round(threads * rounds * 32 / (ms * 1e-3) / 1e12, 3)
+        outs[name] = out
+    torch.cuda.synchronize()
+    res["forms_agree"] = bool(torch.equal(outs["fp64"], outs["shoup32"]) and
+                              torch.equal(outs["fp64"], outs["montgomery32"]))
+    return res

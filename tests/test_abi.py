@@ -89,3 +89,12 @@ def test_datagen(pkg, orc):
         assert all(0 <= v < q for v in vals)
         assert vals[:5] == [0, 1, q - 1, q // 2, q - q // 2]
         np.testing.assert_array_equal(x, dg.operands(1000, q, 5, edge_fraction=0.1, edges_first=True))
+
+
+def test_probe_library_exports_every_declared_symbol(pkg):
+    lib = pkg.probes.load_library()
+    text = (ROOT / "include" / "bytecrt_probes.h").read_text()
+    names = sorted(set(re.findall(r"^\s*(?:[\w\s\*]+?)\b(bcrtp_\w+)\s*\(", text, flags=re.M)))
+    assert names and set(names) == set(pkg.probes.EXPORTED)
+    for n in names:
+        assert hasattr(lib, n), n
