@@ -428,7 +428,7 @@ def cpu_baseline(pkg, q, seconds: float):
 
 def sweep_mode(args, torch):
     """Autotune the encode tile shape, the fusion of encode with the forward
-    columns and then the batch pipelining for the bench geometry (every
+    columns, the kernel flags and then the batch pipelining for the bench geometry (every
     candidate's product is checked bit for bit against the first) and record
     the choices in a pipeline table (copy it into the package to make it the
     default)."""
@@ -449,6 +449,9 @@ def sweep_mode(args, torch):
     fus = tune.tune_fusion(eng, a, b, c, B, reps=args.reps, stream=stream)
     for r in fus["rows"]:
         print(json.dumps(r), flush=True)
+    kfl = tune.tune_kernel_flags(eng, a, b, c, B, reps=args.reps, stream=stream)
+    for r in kfl["rows"]:
+        print(json.dumps(r), flush=True)
     try:
         res = tune.tune_pipeline(eng, a, b, c, B, reps=args.reps, stream=stream)
     except RuntimeError as x:  # the pipeline tuner is switched off
@@ -458,9 +461,9 @@ def sweep_mode(args, torch):
         print(json.dumps(r), flush=True)
     out = Path(args.sweep_out)
     out.parent.mkdir(parents=True, exist_ok=True)
-    tune.record(N, W, B, res, device=torch.cuda.get_device_name(0), path=out, encode=enc, fusion=fus)
+    tune.record(N, W, B, res, device=torch.cuda.get_device_name(0), path=out, encode=enc, fusion=fus, flags=kfl)
     print(json.dumps({"best": res and res["best"], "encode_tile": enc["best"], "fusion": fus["best"],
-                      "table": str(out)}), flush=True)
+                      "kernel_flags": kfl["best"], "table": str(out)}), flush=True)
 
 
 def main():
@@ -509,7 +512,8 @@ def main():
     # -- batched: one graph launch per step, pipelining from the tuned table ---
     tuned = eng.apply_tuning(B)
     pipeline = {"chunk": eng.pipeline[0], "streams": eng.pipeline[1], "encode_tile_ts_log": eng.encode_tile,
-                "fuse_min_tiles": eng.fuse_min_tiles, "source": "table" if tuned else "default"}
+                "fuse_min_tiles": eng.fuse_min_tiles, "kernel_flags": eng.kernel_flags,
+                "source": "table" if tuned else "default"}
     for _ in range(max(args.warmup, 1)):
         eng.run_device(a, b, c, B, stream=stream, graph=True)
     torch.cuda.synchronize()

@@ -33,7 +33,7 @@ EXPORTED = (
     "bcrt_set_encode_tile", "bcrt_set_fusion",
     "bcrt_mix_create", "bcrt_mix_run", "bcrt_mix_last_error", "bcrt_mix_destroy",
     "bcrt_server_create", "bcrt_server_submit", "bcrt_server_wait", "bcrt_server_stats",
-    "bcrt_server_last_error", "bcrt_server_destroy", "bcrt_run_device_noalloc",
+    "bcrt_server_last_error", "bcrt_server_destroy", "bcrt_run_device_noalloc", "bcrt_set_kernel_flags",
 )
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -86,6 +86,8 @@ def load_library():
     lib.bcrt_set_encode_tile.argtypes = [vp, ctypes.c_uint32]
     lib.bcrt_set_fusion.restype = ctypes.c_int
     lib.bcrt_set_fusion.argtypes = [vp, ctypes.c_uint32]
+    lib.bcrt_set_kernel_flags.restype = ctypes.c_int
+    lib.bcrt_set_kernel_flags.argtypes = [vp, ctypes.c_uint32]
     lib.bcrt_plan_host.restype = ctypes.c_int
     lib.bcrt_plan_host.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p,
                                    _u32p, _u32p, _u32p, _u32p, ctypes.c_char_p, ctypes.c_size_t]
@@ -159,6 +161,7 @@ class Engine:
         self.pipeline = (32, 2)  # the library's default (chunk, streams)
         self.encode_tile = min(5, (N // 256).bit_length() - 1)  # the library's default ts_log
         self.fuse_min_tiles = 256  # the library's default fusion threshold
+        self.kernel_flags = 0  # the library's default kernel-level choices
         g = np.zeros(10, dtype=np.uint32)
         lib.bcrt_geometry(self._h, g.ctypes.data_as(_u32p))
         (self.N, self.W, self.L, self.P, self.N1, self.N2, self.Kenc, self.Kcrt, self.Mcrt,
@@ -239,6 +242,15 @@ class Engine:
         ((N/256) * 2 * batch; 0 = whenever N >= 16,384, FUSE_NEVER = never)."""
         self._check(load_library().bcrt_set_fusion(self._h, min_tiles))
         self.fuse_min_tiles = min_tiles
+
+    FLAG_NO_XCD, FLAG_NO_PRIO, FLAG_WEIGHTS_L2 = 1, 2, 4
+
+    def set_kernel_flags(self, flags: int):
+        """Kernel-level choices (bytecrt.h bcrt_set_kernel_flags): FLAG_NO_XCD
+        plain workgroup order, FLAG_NO_PRIO no raised MFMA issue priority in the
+        fused encode, FLAG_WEIGHTS_L2 fused-encode weights from L2."""
+        self._check(load_library().bcrt_set_kernel_flags(self._h, flags))
+        self.kernel_flags = flags
 
     def apply_tuning(self, batch: int, table: dict | None = None) -> bool:
         """Use the pipelining the autotuner recorded for this geometry and batch

@@ -272,6 +272,7 @@ int create_engine(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W,
         t.qshift = e->plan.qshift;
         t.enc_ts_log = std::min(5u, g.n1);
         t.fuse_min_tiles = bcrt::kFusedMinTiles;
+        t.kflags = 0;
         t.qt1 = e->plan.qt1;
         t.inv_qt1 = e->plan.inv_qt1;
         t.primes = upload(*e, e->plan.primes);
@@ -515,6 +516,15 @@ extern "C" int bcrt_set_encode_tile(bcrt_engine* e, uint32_t ts_log) {
             throw std::invalid_argument("ByteCRT encode tile needs 2^ts_log <= min(128, N/256)");
         drop_graph(e);  // captured graphs hold the tables by value
         e->tables.enc_ts_log = ts_log;
+    });
+}
+
+extern "C" int bcrt_set_kernel_flags(bcrt_engine* e, uint32_t flags) {
+    if (!e) return BCRT_EINVAL;
+    return guarded(e, [&] {
+        if (flags > bcrt::kFlagsAll) throw std::invalid_argument("ByteCRT kernel flags must be within 0..7");
+        drop_graph(e);  // captured graphs hold the tables by value
+        e->tables.kflags = flags;
     });
 }
 

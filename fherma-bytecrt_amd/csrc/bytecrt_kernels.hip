@@ -647,7 +647,7 @@ size_t ef_smem_bytes(const DevTables& t) {
     // + one pair of weight tiles (64 rows of Kenc + 16 bytes) when Kenc <= 224;
     // at Kenc = 256 that would leave room for only two workgroups per CU, so
     // that width reads its weight fragments from global memory instead
-    const size_t wtiles = t.Kenc <= 224 ? size_t(64) * (t.Kenc + 16) : 0;
+    const size_t wtiles = t.Kenc <= 224 && !(t.kflags & kFlagWeightsL2) ? size_t(64) * (t.Kenc + 16) : 0;
     return std::max(stage, exch) + sizeof(EncPrime) * kMaxPrimesCrt + wtiles;
 }
 
@@ -666,12 +666,13 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
     // Each pair's two weight tiles are copied into LDS once per workgroup (the
     // four waves then read their fragments there, instead of each loading
     // them from L2), the next pair's during the current transform.
-    constexpr bool kStageW = KS <= 7;
+    // (kFlagWeightsL2 switches the staging off: every wave loads its fragments from L2)
+    const bool stage_w = KS <= 7 && !(t.kflags & kFlagWeightsL2);
     uint8_t* wbuf = reinterpret_cast<uint8_t*>(cst + kMaxPrimesCrt);  // weight tiles of the current pair
     const uint32_t wstride = t.Kenc + 16;
     // copies weight tiles 2 grp, 2 grp + 1 (contiguous in wenc) into wbuf as 16-byte words
     auto load_weights = [&](uint32_t grp) {
-        if constexpr (!kStageW) return;
+        if (!stage_w) return;
         const uint32_t rows = min(64u, (t.PTenc - 2 * grp) * 32), wpr = t.Kenc / 16;
         const uint4* src = reinterpret_cast<const uint4*>(t.wenc + size_t(2 * grp) * 32 * t.Kenc);
         for (uint32_t i = threadIdx.x; i < rows * wpr; i += kThreads) {
@@ -715,13 +716,13 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
         // MFMA and VALU instructions share a SIMD's issue port: a wave in its
         // MFMA block goes first, so the matrix pipe stays fed while the other
         // resident workgroups' transforms fill the gaps (measured 1.5% faster)
-        __builtin_amdgcn_s_setprio(1);
+        if (!(t.kflags & kFlagNoPrio)) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (uint32_t u = 0; u < 2; ++u) {
             const uint32_t pt = 2 * grp + u;
             if (pt < t.PTenc) {
                 v4i A[KS];
-                if constexpr (kStageW) {
+                if (stage_w) {
                     const uint8_t* wrow = wbuf + (u * 32 + row) * wstride + 16 * kh;
 #pragma unroll
                     for (int s2 = 0; s2 < KS; ++s2) A[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
@@ -1084,7 +1085,8 @@ __host__ __device__ inline void stage_grid(const DevTables& t, uint32_t batch, i
 // XCD has its own L2.  Renumbering so that XCD k runs the k-th contiguous
 // eighth of the tiles keeps neighbouring tiles -- which share cache lines of
 // their operands and the prime's twiddles -- in one L2 at about the same time.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t id, uint32_t total) {
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t id, uint32_t total, uint32_t kflags) {
+    if (kflags & kFlagNoXcd) return id;
     const uint32_t per = total / 8, full = per * 8;
     return id < full ? (id % 8) * per + id / 8 : id;
 }
@@ -1102,7 +1104,7 @@ __device__ __forceinline__ void single_xyz(const DevTables& t, int stage, uint32
     uint32_t gx, gy, gz;
     stage_grid(t, 1, stage, &gx, &gy, &gz);
     stage_grid(t, gridDim.x / (gx * gy * gz), stage, &gx, &gy, &gz);  // the launch covers whole products
-    tile_xyz(xcd_tile(blockIdx.x, gridDim.x), gx, gy, x, y, z);
+    tile_xyz(xcd_tile(blockIdx.x, gridDim.x, t.kflags), gx, gy, x, y, z);
     *x = __builtin_amdgcn_readfirstlane(*x);  // uniform: keep them in scalar registers
     *y = __builtin_amdgcn_readfirstlane(*y);
     *z = __builtin_amdgcn_readfirstlane(*z);
@@ -1129,7 +1131,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_encode_fwd(DevTables t, const u
                                                          uint32_t* __restrict__ planes) {
     extern __shared__ __align__(16) uint8_t smem_ef[];
     uint32_t r, op, prod;
-    tile_xyz(xcd_tile(blockIdx.x, gridDim.x), t.N1, 2, &r, &op, &prod);
+    tile_xyz(xcd_tile(blockIdx.x, gridDim.x, t.kflags), t.N1, 2, &r, &op, &prod);
     // uniform over the workgroup: scalar registers, not VGPRs held all kernel
     r = __builtin_amdgcn_readfirstlane(r);
     op = __builtin_amdgcn_readfirstlane(op);
@@ -1186,7 +1188,7 @@ __device__ __forceinline__ bool locate(const GroupTable& g, int stage, uint32_t*
     uint32_t gx, gy, gz;
     stage_grid(g.g[k].t, g.g[k].args.batch, stage, &gx, &gy, &gz);
     const uint32_t count = gx * gy * gz;
-    const uint32_t tile = xcd_tile(blockIdx.x - g.g[k].begin[stage], (count + 7) / 8 * 8);
+    const uint32_t tile = xcd_tile(blockIdx.x - g.g[k].begin[stage], (count + 7) / 8 * 8, g.g[k].t.kflags);
     *k_out = __builtin_amdgcn_readfirstlane(k);  // uniform: scalar registers
     tile_xyz(tile, gx, gy, x, y, z);
     *x = __builtin_amdgcn_readfirstlane(*x);
@@ -1290,7 +1292,7 @@ bool shapes_ok(const DevTables& t, const BatchArgs& args) {
            t.Kcrt % 32 == 0 && t.Mcrt % 32 == 0 && t.P >= 1 && t.P <= kMaxPrimesCrt && t.L >= 3 &&
            t.L <= 64 && t.Kcrt >= 4 * (t.P + 1) && t.Mcrt >= 4 * t.L + 1 && t.Kenc >= 4 * t.L &&
            t.Kcrt <= 32 * 17 && t.Mcrt <= 32 * 4 * crt_tiles_per_wave(t.Kcrt / 32, t.Mcrt / 32) &&
-           t.qshift + 32 == t.W && t.enc_ts_log <= t.n1 && t.enc_ts_log <= 7;
+           t.qshift + 32 == t.W && t.enc_ts_log <= t.n1 && t.enc_ts_log <= 7 && t.kflags <= kFlagsAll;
 }
 
 template <int NLOG1>

@@ -12,6 +12,7 @@ struct DevTables {
     uint32_t N, W, L, P, N1, n1, N2, Kenc, PTenc, Kcrt, Mcrt, qshift;
     uint32_t enc_ts_log;           // encode tile: 2^enc_ts_log consecutive s by 128 >> enc_ts_log r
     uint32_t fuse_min_tiles;       // host: fused encode + forward columns from this many workgroups
+    uint32_t kflags;               // kernel-level choices (kFlag*), 0 = the measured defaults
     uint64_t qt1;                  // (q >> qshift) + 1
     double inv_qt1;                // 1 / qt1
     const uint32_t* primes;        // [P]
@@ -28,6 +29,13 @@ struct DevTables {
     const double* invp;            // [P]
     const uint32_t* q;             // [L]
 };
+
+// Kernel-level choices the autotuner can change at run time (DevTables::kflags).
+// Results never depend on them.
+constexpr uint32_t kFlagNoXcd = 1;      // plain workgroup order instead of the XCD tile renumbering
+constexpr uint32_t kFlagNoPrio = 2;     // fused encode: no raised issue priority over its MFMA block
+constexpr uint32_t kFlagWeightsL2 = 4;  // fused encode: weight fragments from L2, not staged in LDS
+constexpr uint32_t kFlagsAll = 7;
 
 // One batch of products.  a, b, out: [batch][N][L] u32 coefficient-major.
 // planes: [batch][2][P][N] u32 workspace (operand A planes end up holding the

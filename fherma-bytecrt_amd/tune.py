@@ -161,6 +161,43 @@ def tune_fusion(engine, a, b, c, batch: int, reps: int = 5, stream=None):
     return {"best": best, "rows": rows}
 
 
+KERNEL_FLAG_CANDIDATES = tuple(range(8))  # every combination of the three kernel flags
+
+
+def tune_kernel_flags(engine, a, b, c, batch: int, candidates=KERNEL_FLAG_CANDIDATES, reps: int = 3, stream=None):
+    """Time the whole product (the four or five kernels, eager launches: no
+    graph is captured) under each kernel-flag setting (Engine.set_kernel_flags:
+    XCD tile renumbering, the fused encode's MFMA issue priority and its weight
+    staging) for ``batch`` products of device operands a, b (into c).  Checks
+    every setting's product against the first, bit for bit, and leaves the
+    engine on the fastest.  Returns {"best": {...}, "rows": [...]}."""
+    import torch
+
+    if batch < 1 or batch > engine.max_batch:
+        raise ValueError(f"ByteCRT batch must be in [1, max_batch={engine.max_batch}]")
+    stream = stream or torch.cuda.current_stream()
+    ref = None
+    rows = []
+    for flags in candidates:
+        engine.set_kernel_flags(flags)
+        engine.run_stages(31, a=a, b=b, c=c, batch=batch, stream=stream)
+        stream.synchronize()
+        if ref is None:
+            ref = c.clone()
+        elif not torch.equal(c, ref):
+            raise RuntimeError(f"ByteCRT kernel flags {flags} changed the product")
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(reps):
+            engine.run_stages(31, a=a, b=b, c=c, batch=batch, stream=stream)
+        t1.record(stream)
+        t1.synchronize()
+        rows.append({"kernel_flags": flags, "product_ms": round(t0.elapsed_time(t1) / reps, 4)})
+    best = min(rows, key=lambda r: r["product_ms"])
+    engine.set_kernel_flags(best["kernel_flags"])
+    return {"best": best, "rows": rows}
+
+
 def load_table(path: Path | None = None) -> dict:
     path = Path(path or os.environ.get("BCRT_PIPELINE_TABLE", TABLE))
     if not path.exists():
@@ -169,10 +206,10 @@ def load_table(path: Path | None = None) -> dict:
 
 
 def record(N: int, W: int, batch: int, result: dict | None, device: str, path: Path | None = None,
-           encode: dict | None = None, fusion: dict | None = None) -> dict:
-    """Store tune_pipeline, tune_encode_tile and/or tune_fusion results in the
-    table entry for (N, W, batch), keeping what the entry already holds
-    (returns the updated table)."""
+           encode: dict | None = None, fusion: dict | None = None, flags: dict | None = None) -> dict:
+    """Store tune_pipeline, tune_encode_tile, tune_fusion and/or
+    tune_kernel_flags results in the table entry for (N, W, batch), keeping
+    what the entry already holds (returns the updated table)."""
     path = Path(path or os.environ.get("BCRT_PIPELINE_TABLE", TABLE))
     table = load_table(path)
     entry = dict(table.get(_key(N, W, batch), {}))
@@ -184,6 +221,8 @@ def record(N: int, W: int, batch: int, result: dict | None, device: str, path: P
     if fusion is not None:
         entry.update({"fuse_min_tiles": fusion["best"]["fuse_min_tiles"],
                       "encode_fwd_ms": fusion["best"]["encode_fwd_ms"]})
+    if flags is not None:
+        entry.update({"kernel_flags": flags["best"]["kernel_flags"], "product_ms": flags["best"]["product_ms"]})
     entry.update({"device": device, "date": time.strftime("%Y-%m-%d")})
     table[_key(N, W, batch)] = entry
     path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
@@ -211,18 +250,28 @@ def lookup_fusion(N: int, W: int, batch: int, table: dict | None = None):
     return None if e is None or "fuse_min_tiles" not in e else int(e["fuse_min_tiles"])
 
 
+def lookup_kernel_flags(N: int, W: int, batch: int, table: dict | None = None):
+    """The tuned kernel flags for this geometry and batch, or None."""
+    table = load_table() if table is None else table
+    e = table.get(_key(N, W, batch))
+    return None if e is None or "kernel_flags" not in e else int(e["kernel_flags"])
+
+
 def apply(engine, batch: int, table: dict | None = None) -> bool:
     """Put the engine on its tuned settings for ``batch`` products (pipelining,
-    encode tile and fusion, whichever the table holds); returns whether it
-    had any."""
+    encode tile, fusion and kernel flags, whichever the table holds); returns
+    whether it had any."""
     table = load_table() if table is None else table
     hit = lookup(engine.N, engine.W, batch, table)
     ts = lookup_encode_tile(engine.N, engine.W, batch, table)
     fu = lookup_fusion(engine.N, engine.W, batch, table)
+    kf = lookup_kernel_flags(engine.N, engine.W, batch, table)
+    if kf is not None:
+        engine.set_kernel_flags(kf)
     if hit is not None:
         engine.set_pipeline(*hit)
     if ts is not None:
         engine.set_encode_tile(ts)
     if fu is not None:
         engine.set_fusion(fu)
-    return hit is not None or ts is not None or fu is not None
+    return hit is not None or ts is not None or fu is not None or kf is not None

@@ -113,6 +113,17 @@ int bcrt_set_encode_tile(bcrt_engine* e, uint32_t ts_log);
  * it.  Applies to later runs and later mixes. */
 int bcrt_set_fusion(bcrt_engine* e, uint32_t min_tiles);
 
+/* Kernel-level choices (the autotuner's third knob; no reference counterpart:
+ * the reference fixes its kernels' tiling and scheduling at compile time,
+ * ntt-pipeline.cuh:14-15, rns-encode-mma.cuh:148-252).  Bits: 1 launches the
+ * workgroups in plain order instead of renumbering the tiles so that each
+ * XCD (and its L2) runs one contiguous eighth; 2 drops the raised issue
+ * priority the fused encode gives its MFMA blocks; 4 makes the fused encode's
+ * waves load their weight fragments from L2 instead of staging each pair of
+ * weight tiles in LDS.  Default 0.  Results do not depend on them.  Applies
+ * to later runs and later mixes. */
+int bcrt_set_kernel_flags(bcrt_engine* e, uint32_t flags);
+
 /* ---- Mixed batches (products of different N, W, q in one launch per stage) ----
  * No reference counterpart: the reference runs one fixed geometry per process
  * (native-batched/solve.cu:163-164).  Each group names an engine (its plan

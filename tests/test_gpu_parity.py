@@ -445,3 +445,36 @@ def test_fherma_mirror_roundtrip(pkg, orc, gu):
     with pytest.raises(RuntimeError, match="coefficient count"):
         bc.fherma_run(st, bc.Inputs(a=bc.Tensor([1], a.reshape(-1)[:5]), b=bc.Tensor([1], b.reshape(-1)[:5])))
     bc.fherma_free(st)
+
+
+@pytest.mark.parametrize("N,W,B", [(65536, 1747, 2), (16384, 436, 3), (8192, 220, 1)])
+def test_tune_kernel_flags_exact(pkg, orc, gu, tmp_path, N, W, B):
+    """Every kernel-flag setting (XCD tile renumbering, the fused encode's
+    MFMA priority and weight staging) gives the same products, bit for bit;
+    the tuner leaves the engine on the fastest, records it, and a fresh engine
+    that applies the table computes the same products (eager and graph).
+    Tables built at N = 8,192 exercise the separate encode kernel and the
+    plain grids under the flags too."""
+    import importlib
+    import torch
+    tune = importlib.import_module("bytecrt_amd.tune")
+    q = pkg.datagen.modulus(W, W + 5)
+    a = np.stack([pkg.datagen.operands(N, q, 30 + i, edge_fraction=0.1) for i in range(B)])
+    b = np.stack([pkg.datagen.operands(N, q, 40 + i) for i in range(B)])
+    e = pkg.Engine(N, W, q, max_batch=B)
+    da, db = gu.to_dev(a).view(B, N, -1), gu.to_dev(b).view(B, N, -1)
+    out = torch.empty_like(da)
+    res = tune.tune_kernel_flags(e, da, db, out, B, reps=1)
+    assert [r["kernel_flags"] for r in res["rows"]] == list(range(8))
+    assert e.kernel_flags == res["best"]["kernel_flags"]
+    got = gu.from_dev(out).reshape(B, N, e.L)
+    np.testing.assert_array_equal(got[0], orc.Plan(N, W, q).product(a[0], b[0]))
+    path = tmp_path / "table.json"
+    tune.record(N, W, B, None, device="test", path=path, flags=res)
+    f = pkg.Engine(N, W, q, max_batch=B)
+    assert f.apply_tuning(B, tune.load_table(path)) and f.kernel_flags == e.kernel_flags
+    for graph in (False, True):
+        out2 = torch.zeros_like(out)
+        f.run_device(da, db, out2, B, graph=graph)
+        gu.sync()
+        assert torch.equal(out2, out)

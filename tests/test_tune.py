@@ -32,6 +32,9 @@ class FakeEngine:
     def set_fusion(self, min_tiles):
         self.calls.append(("fuse", min_tiles))
 
+    def set_kernel_flags(self, flags):
+        self.calls.append(("flags", flags))
+
 
 def test_record_lookup_apply(tune, tmp_path):
     path = tmp_path / "table.json"
@@ -112,3 +115,20 @@ def test_pipeline_tuner_is_switched_off(tune):
     an eager run crashed the host process twice, cause not established)."""
     with pytest.raises(RuntimeError, match="switched off"):
         tune.tune_pipeline(None, None, None, None, 1)
+
+
+def test_kernel_flag_entries(tune, tmp_path):
+    """Kernel flags are kept beside the other choices of an entry and applied
+    with them; an entry without them leaves the engine's flags alone."""
+    path = tmp_path / "table.json"
+    fus = {"best": {"fuse_min_tiles": 0, "structure": "fused", "encode_fwd_ms": 5.8}, "rows": []}
+    flg = {"best": {"kernel_flags": 2, "product_ms": 14.6}, "rows": []}
+    tune.record(65536, 1747, 128, None, device="test", path=path, fusion=fus)
+    tune.record(65536, 1747, 128, None, device="test", path=path, flags=flg)
+    table = tune.load_table(path)
+    assert tune.lookup_kernel_flags(65536, 1747, 128, table) == 2
+    assert tune.lookup_fusion(65536, 1747, 128, table) == 0
+    e = FakeEngine(65536, 1747)
+    assert tune.apply(e, 128, table) and ("flags", 2) in e.calls and ("fuse", 0) in e.calls
+    assert tune.lookup_kernel_flags(32768, 868, 128, table) is None
+    assert tune.KERNEL_FLAG_CANDIDATES == tuple(range(8))
