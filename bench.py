@@ -270,13 +270,18 @@ def uneven_leg(pkg, torch, steps: int):
     groups[3][0].run_device(groups[3][1][5], groups[3][2][5], one, 1, stream=stream)
     torch.cuda.synchronize()
     exact = exact and bool(torch.equal(one, got[3][5]))
+    # every candidate schedule, each checked bit for bit by the tuner
+    import bytecrt_amd.tune as tune
+    sweep = tune.tune_mix(mix, [g[3] for g in groups], reps=steps, stream=stream)
+    exact = exact and all(bool(torch.equal(g[3], r)) for g, r in zip(groups, got))
     total = sum(g[4] for g in groups)
     mix.close()
     return {"workload": "96 x (N=32768, W=868) + 32 x (N=65536, W=1747), 3 moduli per geometry, "
-                        "10% edge coefficients, one launch per stage (hipGraph)",
+                        "10% edge coefficients, one launch per stage and geometry class (hipGraph)",
             "products": total, "mixed_ms": round(mixed_ms, 3),
             "mixed_products_per_s": round(total / mixed_ms * 1e3, 1),
-            "separate_launches_ms": round(separate_ms, 3), "exact": exact}
+            "separate_launches_ms": round(separate_ms, 3), "exact": exact,
+            "schedule_sweep": sweep["rows"]}
 
 
 SERVE_GEOMS = [(8192, 220), (16384, 436), (32768, 868), (65536, 1747)]

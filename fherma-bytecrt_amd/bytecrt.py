@@ -31,7 +31,8 @@ EXPORTED = (
     "bcrt_synchronize", "bcrt_last_error", "bcrt_geometry", "bcrt_plan_tables",
     "bcrt_run_stages", "bcrt_planes_words", "bcrt_mfma_probe", "bcrt_plan_host", "bcrt_set_pipeline",
     "bcrt_set_encode_tile", "bcrt_set_fusion",
-    "bcrt_mix_create", "bcrt_mix_run", "bcrt_mix_last_error", "bcrt_mix_destroy",
+    "bcrt_mix_create", "bcrt_mix_run", "bcrt_mix_last_error", "bcrt_mix_destroy", "bcrt_mix_set_schedule",
+    "bcrt_mix_chunks",
     "bcrt_server_create", "bcrt_server_submit", "bcrt_server_wait", "bcrt_server_stats",
     "bcrt_server_last_error", "bcrt_server_destroy", "bcrt_run_device_noalloc", "bcrt_set_kernel_flags",
 )
@@ -99,6 +100,10 @@ def load_library():
     lib.bcrt_mix_last_error.restype = ctypes.c_char_p
     lib.bcrt_mix_last_error.argtypes = [vp]
     lib.bcrt_mix_destroy.argtypes = [vp]
+    lib.bcrt_mix_set_schedule.restype = ctypes.c_int
+    lib.bcrt_mix_set_schedule.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+    lib.bcrt_mix_chunks.restype = ctypes.c_int
+    lib.bcrt_mix_chunks.argtypes = [vp]
     lib.bcrt_server_create.restype = ctypes.c_int
     lib.bcrt_server_create.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t, ctypes.c_uint32,
                                        ctypes.c_char_p, ctypes.c_size_t]
@@ -264,7 +269,8 @@ class Engine:
 
 class Mix:
     """Products of several engines (different N, W, q) run as one launch per
-    stage.  ``groups`` is a list of (engine, d_a, d_b, d_c, batch)."""
+    stage and geometry class under the default schedule (set_schedule changes
+    it).  ``groups`` is a list of (engine, d_a, d_b, d_c, batch)."""
 
     def __init__(self, groups):
         lib = load_library()
@@ -280,12 +286,29 @@ class Mix:
         if rc != BCRT_OK:
             _raise(rc, err.value.decode())
         self._h = handle
+        self.schedule = (1, 1, 0)
+
+    MERGE_CLASSES = 1
 
     def run(self, stream=None, graph: bool = False):
         lib = load_library()
         rc = lib.bcrt_mix_run(self._h, _stream_handle(stream), 1 if graph else 0)
         if rc != BCRT_OK:
             _raise(rc, lib.bcrt_mix_last_error(self._h).decode())
+
+    def set_schedule(self, chunks: int = 1, streams: int = 1, flags: int = 0):
+        """Chunks per geometry class, worker streams (1: all chunks on the
+        caller's stream) and flags (MERGE_CLASSES), each count in 1..16;
+        results are unchanged."""
+        lib = load_library()
+        rc = lib.bcrt_mix_set_schedule(self._h, chunks, streams, flags)
+        if rc != BCRT_OK:
+            _raise(rc, lib.bcrt_mix_last_error(self._h).decode())
+        self.schedule = (chunks, streams, flags)
+
+    @property
+    def chunks(self) -> int:
+        return int(load_library().bcrt_mix_chunks(self._h))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -540,13 +540,21 @@ extern "C" int bcrt_set_fusion(bcrt_engine* e, uint32_t min_tiles) {
 // Mixed batches
 // ---------------------------------------------------------------------------
 // Groups are classed by geometry (N, W); the products of each class are dealt
-// into up to kMixChunks chunks (each group split evenly).  A chunk is one
-// mixed launch per stage over the class's moduli, and chunks run on their own
-// worker streams, so stages of different chunks and classes overlap.
-constexpr uint32_t kMixChunks = 4;
-constexpr uint32_t kMixChunkMin = 8;  // products per chunk before splitting further
+// into chunks (each group split evenly).  A chunk is one mixed launch per
+// stage over its products, and chunks go round robin onto worker streams, so
+// stages of different chunks and classes may overlap.  The schedule (chunks
+// per class, streams, or every class in the same chunks) is set per mix;
+// results do not depend on it.  The default, one chunk per class and one
+// stream, was the fastest measured at 96 x (N=32,768) + 32 x (N=65,536)
+// products: the kernels fill the device by themselves, and concurrent
+// chunks only share it.
+constexpr uint32_t kMixMaxChunks = 16;  // chunks per class, streams a schedule may ask for
 
 struct bcrt_mix {
+    std::vector<bcrt_group> groups;  // as created
+    uint32_t sched_chunks = 1;       // chunks per class
+    uint32_t sched_streams = 1;      // worker streams (at most one per chunk)
+    uint32_t sched_flags = 0;        // BCRT_MIX_MERGE_CLASSES
     std::vector<std::vector<bcrt::GroupDesc>> chunks;
     std::vector<bcrt_engine*> engines;  // referenced (mix_refs) for the mix's lifetime
     hipStream_t stream = nullptr;
@@ -561,42 +569,113 @@ struct bcrt_mix {
 
 namespace {
 
-// Waits for the mix's last launches, then frees its graph, events and streams.
-void release_mix(bcrt_mix* m) {
+// Waits for the mix's last launches, then frees its graph, fork/join events
+// and worker streams (the layout of one schedule).
+void drop_layout(bcrt_mix* m) {
     if (m->graph_launched) (void)hipEventSynchronize(m->graph_done);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (auto st : m->workers) (void)hipStreamSynchronize(st);
-    if (m->graph_done) (void)hipEventDestroy(m->graph_done);
-    m->graph_done = nullptr;
     m->graph_launched = false;
     if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->graph_exec = nullptr;
+    m->graph = nullptr;
     destroy_events(m->eager_events);
     destroy_events(m->graph_events);
     for (auto st : m->workers) (void)hipStreamDestroy(st);
-    if (m->stream) (void)hipStreamDestroy(m->stream);
     m->workers.clear();
-    m->graph_exec = nullptr;
-    m->graph = nullptr;
+    m->chunks.clear();
+}
+
+// drop_layout, then the mix's own stream and event.
+void release_mix(bcrt_mix* m) {
+    drop_layout(m);
+    if (m->graph_done) (void)hipEventDestroy(m->graph_done);
+    m->graph_done = nullptr;
+    if (m->stream) (void)hipStreamDestroy(m->stream);
     m->stream = nullptr;
 }
 
-// Enqueue every chunk on `origin` (forked over the worker streams with the
-// events of `ev` when there are several; also valid inside a stream capture).
+// Deals the mix's groups into chunks under its schedule and creates the
+// worker streams and events they need.
+void layout_mix(bcrt_mix* m) {
+    const auto& groups = m->groups;
+    const uint32_t ngroups = uint32_t(groups.size());
+    // Groups of one geometry (same kernel resource needs) share chunks unless
+    // the schedule merges every class into the same chunks.
+    std::vector<std::vector<uint32_t>> classes;
+    for (uint32_t k = 0; k < ngroups; ++k) {
+        const auto& t = groups[k].engine->tables;
+        bool placed = false;
+        for (auto& cl : classes) {
+            const auto& u = groups[cl[0]].engine->tables;
+            if ((m->sched_flags & BCRT_MIX_MERGE_CLASSES) || (u.N == t.N && u.W == t.W)) {
+                cl.push_back(k);
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) classes.push_back({k});
+    }
+    for (const auto& cl : classes) {
+        uint32_t in_class = 0;
+        for (uint32_t k : cl) in_class += groups[k].batch;
+        const uint32_t nchunks = std::min(m->sched_chunks, in_class);
+        for (uint32_t c = 0; c < nchunks; ++c) {
+            std::vector<bcrt::GroupDesc> chunk;
+            uint32_t begin[bcrt::kGroupStages] = {};
+            for (uint32_t k : cl) {
+                const bcrt_group& g = groups[k];
+                const uint32_t n = g.batch / nchunks + (c < g.batch % nchunks ? 1 : 0);
+                if (n == 0) continue;
+                const uint32_t first = c * (g.batch / nchunks) + std::min(c, g.batch % nchunks);
+                const auto& t = g.engine->tables;
+                const size_t io = size_t(t.N) * t.L, pl = size_t(2) * t.P * t.N;
+                bcrt::GroupDesc d{};
+                d.t = t;
+                d.args = bcrt::BatchArgs{g.d_a + first * io, g.d_b + first * io, g.d_c + first * io,
+                                         g.engine->planes + first * pl, n};
+                for (int st = 0; st < bcrt::kGroupStages; ++st) {
+                    d.begin[st] = begin[st];
+                    begin[st] += bcrt::group_workgroups(d.t, n, st);
+                }
+                chunk.push_back(d);
+            }
+            if (!chunk.empty()) m->chunks.push_back(std::move(chunk));
+        }
+    }
+    const uint32_t nchunks = uint32_t(m->chunks.size());
+    const uint32_t nstreams = std::min(m->sched_streams, nchunks);
+    if (nstreams > 1) {
+        for (uint32_t c = 0; c < nstreams; ++c) {
+            hipStream_t st = nullptr;
+            check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "create mix worker stream");
+            m->workers.push_back(st);
+        }
+        ensure_events(m->eager_events, nstreams);
+        ensure_events(m->graph_events, nstreams);
+    }
+}
+
+// Enqueue every chunk on `origin`, or forked over the worker streams with the
+// events of `ev` when the schedule has several (also valid inside a stream
+// capture).
 void enqueue_mix(bcrt_mix* m, hipStream_t origin, EventSet& ev) {
-    if (m->chunks.size() == 1) {
-        check(bcrt::launch_product_groups(m->chunks[0].data(), uint32_t(m->chunks[0].size()), origin),
-              "launch mixed batch");
+    if (m->workers.empty()) {
+        for (const auto& chunk : m->chunks)
+            check(bcrt::launch_product_groups(chunk.data(), uint32_t(chunk.size()), origin), "launch mixed batch");
         return;
     }
-    ensure_events(ev, m->chunks.size());
+    const size_t ns = m->workers.size();
+    ensure_events(ev, ns);
     check(hipEventRecord(ev.fork, origin), "fork");
-    for (size_t c = 0; c < m->chunks.size(); ++c) {
-        check(hipStreamWaitEvent(m->workers[c], ev.fork, 0), "fork wait");
-        check(bcrt::launch_product_groups(m->chunks[c].data(), uint32_t(m->chunks[c].size()), m->workers[c]),
+    for (size_t w = 0; w < ns; ++w) check(hipStreamWaitEvent(m->workers[w], ev.fork, 0), "fork wait");
+    for (size_t c = 0; c < m->chunks.size(); ++c)
+        check(bcrt::launch_product_groups(m->chunks[c].data(), uint32_t(m->chunks[c].size()), m->workers[c % ns]),
               "launch mixed chunk");
-        check(hipEventRecord(ev.joins[c], m->workers[c]), "join record");
-        check(hipStreamWaitEvent(origin, ev.joins[c], 0), "join wait");
+    for (size_t w = 0; w < ns; ++w) {
+        check(hipEventRecord(ev.joins[w], m->workers[w]), "join record");
+        check(hipStreamWaitEvent(origin, ev.joins[w], 0), "join wait");
     }
 }
 
@@ -622,62 +701,11 @@ extern "C" int bcrt_mix_create(bcrt_mix** out, const bcrt_group* groups, uint32_
             require_batch(g.engine, g.batch);
             total += g.batch;
         }
-        // Groups of one geometry (same kernel resource needs) share chunks;
-        // each geometry class is dealt into up to kMixChunks chunks.
-        std::vector<std::vector<uint32_t>> classes;
-        for (uint32_t k = 0; k < ngroups; ++k) {
-            const auto& t = groups[k].engine->tables;
-            bool placed = false;
-            for (auto& cl : classes) {
-                const auto& u = groups[cl[0]].engine->tables;
-                if (u.N == t.N && u.W == t.W) {
-                    cl.push_back(k);
-                    placed = true;
-                    break;
-                }
-            }
-            if (!placed) classes.push_back({k});
-        }
         (void)total;
-        for (const auto& cl : classes) {
-            uint32_t in_class = 0;
-            for (uint32_t k : cl) in_class += groups[k].batch;
-            const uint32_t nchunks = std::max(1u, std::min(kMixChunks, in_class / kMixChunkMin));
-            for (uint32_t c = 0; c < nchunks; ++c) {
-                std::vector<bcrt::GroupDesc> chunk;
-                uint32_t begin[bcrt::kGroupStages] = {};
-                for (uint32_t k : cl) {
-                    const bcrt_group& g = groups[k];
-                    const uint32_t n = g.batch / nchunks + (c < g.batch % nchunks ? 1 : 0);
-                    if (n == 0) continue;
-                    const uint32_t first = c * (g.batch / nchunks) + std::min(c, g.batch % nchunks);
-                    const auto& t = g.engine->tables;
-                    const size_t io = size_t(t.N) * t.L, pl = size_t(2) * t.P * t.N;
-                    bcrt::GroupDesc d{};
-                    d.t = t;
-                    d.args = bcrt::BatchArgs{g.d_a + first * io, g.d_b + first * io, g.d_c + first * io,
-                                             g.engine->planes + first * pl, n};
-                    for (int st = 0; st < bcrt::kGroupStages; ++st) {
-                        d.begin[st] = begin[st];
-                        begin[st] += bcrt::group_workgroups(d.t, n, st);
-                    }
-                    chunk.push_back(d);
-                }
-                m->chunks.push_back(std::move(chunk));
-            }
-        }
-        const uint32_t nchunks = uint32_t(m->chunks.size());
+        m->groups.assign(groups, groups + ngroups);
         check(hipStreamCreate(&m->stream), "create mix stream");
         check(hipEventCreateWithFlags(&m->graph_done, hipEventDisableTiming), "create graph event");
-        if (nchunks > 1) {
-            for (uint32_t c = 0; c < nchunks; ++c) {
-                hipStream_t st = nullptr;
-                check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "create mix worker stream");
-                m->workers.push_back(st);
-            }
-            ensure_events(m->eager_events, nchunks);
-            ensure_events(m->graph_events, nchunks);
-        }
+        layout_mix(m);
     } catch (const std::invalid_argument& x) {
         rc = BCRT_EINVAL;
         m->last_error = x.what();
@@ -745,6 +773,30 @@ extern "C" int bcrt_mix_run(bcrt_mix* m, void* stream, int graph) {
 }
 
 extern "C" const char* bcrt_mix_last_error(const bcrt_mix* m) { return m ? m->last_error.c_str() : "null mix"; }
+
+extern "C" int bcrt_mix_set_schedule(bcrt_mix* m, uint32_t chunks, uint32_t streams, uint32_t flags) {
+    if (!m) return BCRT_EINVAL;
+    if (chunks < 1 || chunks > kMixMaxChunks || streams < 1 || streams > kMixMaxChunks ||
+        (flags & ~uint32_t(BCRT_MIX_MERGE_CLASSES))) {
+        m->last_error = "ByteCRT mix schedule needs 1 <= chunks, streams <= " + std::to_string(kMixMaxChunks) +
+                        " and flags within BCRT_MIX_MERGE_CLASSES";
+        return BCRT_EINVAL;
+    }
+    try {
+        drop_layout(m);
+        m->sched_chunks = chunks;
+        m->sched_streams = streams;
+        m->sched_flags = flags;
+        layout_mix(m);
+        m->last_error.clear();
+        return BCRT_OK;
+    } catch (const std::exception& x) {
+        m->last_error = x.what();
+        return BCRT_ERUNTIME;
+    }
+}
+
+extern "C" int bcrt_mix_chunks(const bcrt_mix* m) { return m ? int(m->chunks.size()) : BCRT_EINVAL; }
 
 extern "C" void bcrt_mix_destroy(bcrt_mix* m) {
     if (!m) return;

@@ -1205,7 +1205,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_encode_groups(const GroupTable 
     uint32_t x, y, z;
     uint32_t k;
     if (!locate(g, 0, &k, &x, &y, &z)) return;
-    const GroupDesc& d = g.g[k];
+    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
     if constexpr (KS == 0)
         encode_dispatch(d.t, d.args.a, d.args.b, d.args.planes, x, y, z, smem_enc_g);
     else
@@ -1217,7 +1217,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_encode_fwd_groups(const GroupTa
     extern __shared__ __align__(16) uint8_t smem_ef_g[];
     uint32_t r, op, prod, k;
     if (!locate(g, 5, &k, &r, &op, &prod)) return;
-    const GroupDesc& d = g.g[k];
+    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
     encfwd_wg<KS, NLOG1>(d.t, d.args.a, d.args.b, d.args.planes, __builtin_amdgcn_readfirstlane(r),
                   __builtin_amdgcn_readfirstlane(op), __builtin_amdgcn_readfirstlane(prod), smem_ef_g);
 }
@@ -1228,7 +1228,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_cols_groups(const GroupTable g
     uint32_t x, y, z;
     uint32_t k;
     if (!locate(g, 1, &k, &x, &y, &z)) return;
-    const GroupDesc& d = g.g[k];
+    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
     double* s = reinterpret_cast<double*>(smem_fc_g);
     if constexpr (NLOG1 != 0) {
         fwd_cols_wg<NLOG1>(d.t, d.args.planes, x, y, z, s);
@@ -1248,7 +1248,7 @@ __global__ __launch_bounds__(kThreads) void k_middle_groups(const GroupTable g) 
     uint32_t x, y, z;
     uint32_t k;
     if (!locate(g, 2, &k, &x, &y, &z)) return;
-    const GroupDesc& d = g.g[k];
+    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
     middle_wg(d.t, d.args.planes, x, y, z, reinterpret_cast<double*>(smem_mid_g));
 }
 
@@ -1258,7 +1258,7 @@ __global__ __launch_bounds__(kThreads) void k_inv_cols_groups(const GroupTable g
     uint32_t x, y, z;
     uint32_t k;
     if (!locate(g, 3, &k, &x, &y, &z)) return;
-    const GroupDesc& d = g.g[k];
+    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
     double* s = reinterpret_cast<double*>(smem_ic_g);
     if constexpr (NLOG1 != 0) {
         inv_cols_wg<NLOG1>(d.t, d.args.planes, x, z, s);
@@ -1279,7 +1279,7 @@ __global__ __launch_bounds__(kCrtThreads, 5) void k_crt_groups(const GroupTable 
     uint32_t x, y, z;
     uint32_t k;
     if (!locate(g, 4, &k, &x, &y, &z)) return;
-    const GroupDesc& d = g.g[k];
+    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
     if constexpr (KS == 0)
         crt_dispatch(d.t, d.args.planes, d.args.out, x, z, smem_crt_g);
     else

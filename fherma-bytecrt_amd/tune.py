@@ -198,6 +198,46 @@ def tune_kernel_flags(engine, a, b, c, batch: int, candidates=KERNEL_FLAG_CANDID
     return {"best": best, "rows": rows}
 
 
+# Mix schedules (chunks per geometry class, worker streams, flags): one launch
+# sequence per class on the caller's stream, classes concurrently on two
+# streams, chunked classes over two to eight streams, and MERGE_CLASSES (every
+# class in the same launches, the kernels switching shape per workgroup).
+MIX_CANDIDATES = ((1, 1, 0), (1, 2, 0), (2, 2, 0), (2, 4, 0), (4, 4, 0), (4, 8, 0), (1, 1, 1))
+
+
+def tune_mix(mix, outputs, candidates=MIX_CANDIDATES, reps: int = 5, stream=None):
+    """Time the mix's graph replay under each schedule (Mix.set_schedule) and
+    leave it on the fastest.  ``outputs`` are the mix's result tensors (one
+    per group); every schedule's results are checked against the first
+    schedule's, bit for bit.  Returns {"best": {...}, "rows": [...]}."""
+    import torch
+
+    stream = stream or torch.cuda.current_stream()
+    ref = None
+    rows = []
+    for chunks, streams, flags in candidates:
+        mix.set_schedule(chunks, streams, flags)
+        for c in outputs:
+            c.zero_()
+        mix.run(stream=stream, graph=True)
+        stream.synchronize()
+        if ref is None:
+            ref = [c.clone() for c in outputs]
+        elif not all(torch.equal(c, r) for c, r in zip(outputs, ref)):
+            raise RuntimeError(f"ByteCRT mix schedule {(chunks, streams, flags)} changed a product")
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(reps):
+            mix.run(stream=stream, graph=True)
+        t1.record(stream)
+        t1.synchronize()
+        rows.append({"chunks": chunks, "streams": streams, "flags": flags, "launches": mix.chunks,
+                     "mix_ms": round(t0.elapsed_time(t1) / reps, 4)})
+    best = min(rows, key=lambda r: r["mix_ms"])
+    mix.set_schedule(best["chunks"], best["streams"], best["flags"])
+    return {"best": best, "rows": rows}
+
+
 def load_table(path: Path | None = None) -> dict:
     path = Path(path or os.environ.get("BCRT_PIPELINE_TABLE", TABLE))
     if not path.exists():

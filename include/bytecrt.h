@@ -124,13 +124,13 @@ int bcrt_set_fusion(bcrt_engine* e, uint32_t min_tiles);
  * to later runs and later mixes. */
 int bcrt_set_kernel_flags(bcrt_engine* e, uint32_t flags);
 
-/* ---- Mixed batches (products of different N, W, q in one launch per stage) ----
+/* ---- Mixed batches (products of different N, W, q, one launch per stage) ----
  * No reference counterpart: the reference runs one fixed geometry per process
  * (native-batched/solve.cu:163-164).  Each group names an engine (its plan
  * and workspace; engines must be distinct; at most 16 groups) and a device
- * batch for it.  The
- * five stages each run as ONE launch whose workgroups cover every product of
- * every group. */
+ * batch for it.  Under the default schedule (bcrt_mix_set_schedule) the
+ * five stages of each geometry class run as ONE launch each, whose
+ * workgroups cover every product of every modulus of that class. */
 typedef struct bcrt_group {
     bcrt_engine* engine;
     const uint32_t* d_a;  /* [batch][N][L] device */
@@ -149,6 +149,23 @@ int bcrt_mix_create(bcrt_mix** out, const bcrt_group* groups, uint32_t ngroups, 
 int bcrt_mix_run(bcrt_mix* m, void* stream, int graph);
 const char* bcrt_mix_last_error(const bcrt_mix* m);
 void bcrt_mix_destroy(bcrt_mix* m);
+
+/* Schedule of a mix (no reference counterpart; the reference's batched driver
+ * fixes one: 4 slots over 3 streams, native-batched/solve.cu:17-18).  Groups
+ * of one geometry (N, W) form a class; each class is dealt into `chunks`
+ * chunks (each group split evenly), a chunk being one launch per stage over
+ * its products, and chunks go round robin onto `streams` worker streams
+ * forked from the caller's stream (1: every chunk on the caller's stream, in
+ * order).  BCRT_MIX_MERGE_CLASSES puts every class into the same chunks (one
+ * launch per stage and chunk over all geometries; the kernels then switch
+ * shape per workgroup).  1 <= chunks, streams <= 16; a new mix has chunks =
+ * streams = 1, no flags.  Waits for the mix's pending work and drops its
+ * graph; results do not depend on the schedule. */
+#define BCRT_MIX_MERGE_CLASSES 1u
+int bcrt_mix_set_schedule(bcrt_mix* m, uint32_t chunks, uint32_t streams, uint32_t flags);
+/* Launch sequences (chunks) of the mix's current schedule; BCRT_EINVAL for a
+ * null mix. */
+int bcrt_mix_chunks(const bcrt_mix* m);
 
 /* ---- Serving: continuous batching of single-product requests ----
  * No reference counterpart (the reference serves one fixed-size batch per

@@ -89,10 +89,12 @@ def test_mix_fused_encode_class(pkg, gu):
     mix.close()
 
 
-def test_mix_chunked_over_streams(pkg, orc, gu):
-    """26 products: the mix is dealt into three chunks on worker streams (each
-    group split across chunks); eager and graph replays equal each product
-    computed alone."""
+def test_mix_schedules(pkg, orc, gu):
+    """26 products of three geometries under every schedule shape: one launch
+    sequence per class on the caller's stream (the default), classes dealt
+    into chunks on worker streams (each group split across chunks), chunks
+    sharing fewer streams, and every class merged into the same launches;
+    eager and graph replays equal each product computed alone."""
     spec = [(8192, 220, 12), (4096, 100, 9), (16384, 436, 5)]
     groups, host = [], []
     for k, (N, W, B) in enumerate(spec):
@@ -104,19 +106,62 @@ def test_mix_chunked_over_streams(pkg, orc, gu):
         groups.append((e, gu.to_dev(a), gu.to_dev(b), out, B))
         host.append((e, q, a, b, out))
     mix = pkg.Mix(groups)
+    assert mix.schedule == (1, 1, 0) and mix.chunks == 3  # one launch sequence per geometry class
     want = []
     for e, q, a, b, out in host:
         want.append(np.stack([alone(pkg, gu, e, a[i], b[i]) for i in range(a.shape[0])]))
-    for graph in (False, True, True):
-        for *_, out in host:
-            out.zero_()
-        gu.sync()
-        mix.run(graph=graph)
-        gu.sync()
-        for (e, *_r, out), w in zip(host, want):
-            np.testing.assert_array_equal(gu.from_dev(out).reshape(-1, e.N, e.L), w, err_msg=f"N={e.N}")
+    # (chunks per class, streams, flags) -> launch sequences: 5 products cap the N=16,384 class
+    for sched, launches in (((1, 1, 0), 3), ((3, 3, 0), 9), ((4, 2, 0), 12), ((2, 1, 0), 6),
+                            ((1, 1, pkg.Mix.MERGE_CLASSES), 1), ((3, 2, pkg.Mix.MERGE_CLASSES), 3)):
+        mix.set_schedule(*sched)
+        assert mix.chunks == launches, sched
+        for graph in (False, True, True):
+            for *_, out in host:
+                out.zero_()
+            gu.sync()
+            mix.run(graph=graph)
+            gu.sync()
+            for (e, *_r, out), w in zip(host, want):
+                np.testing.assert_array_equal(gu.from_dev(out).reshape(-1, e.N, e.L), w,
+                                              err_msg=f"N={e.N} schedule={sched} graph={graph}")
+    for bad in ((0, 1, 0), (17, 1, 0), (1, 0, 0), (1, 17, 0), (1, 1, 2)):
+        with pytest.raises(ValueError):
+            mix.set_schedule(*bad)
     e, q, a, b, _ = host[2]
     np.testing.assert_array_equal(want[2][4], orc.Plan(e.N, e.W, q).product(a[4], b[4]))
+    mix.close()
+
+
+def test_tune_mix_exact(pkg, gu):
+    """The mix tuner times every candidate schedule, checks each one's
+    products bit for bit and leaves the mix on the fastest; the tuned mix
+    still equals each product computed alone."""
+    from importlib import import_module
+    tune = import_module("bytecrt_amd.tune")
+    spec = [(8192, 220, 6, 311), (16384, 436, 3, 312), (8192, 220, 2, 313)]
+    groups, host = [], []
+    for N, W, B, seed in spec:
+        q = pkg.datagen.modulus(W, seed)
+        e = pkg.Engine(N, W, q, max_batch=B)
+        a = np.stack([pkg.datagen.operands(N, q, 10 * seed + i, edge_fraction=0.1) for i in range(B)])
+        b = np.stack([pkg.datagen.operands(N, q, 10 * seed + 5 + i) for i in range(B)])
+        out = gu.empty_dev(B * N * e.L)
+        groups.append((e, gu.to_dev(a), gu.to_dev(b), out, B))
+        host.append((e, a, b, out))
+    mix = pkg.Mix(groups)
+    res = tune.tune_mix(mix, [g[3] for g in groups], reps=2)
+    assert len(res["rows"]) == len(tune.MIX_CANDIDATES)
+    best = res["best"]
+    assert mix.schedule == (best["chunks"], best["streams"], best["flags"])
+    for *_, out in host:
+        out.zero_()
+    gu.sync()
+    mix.run(graph=True)
+    gu.sync()
+    for e, a, b, out in host:
+        got = gu.from_dev(out).reshape(-1, e.N, e.L)
+        for i in range(got.shape[0]):
+            np.testing.assert_array_equal(got[i], alone(pkg, gu, e, a[i], b[i]), err_msg=f"N={e.N} i={i}")
     mix.close()
 
 
