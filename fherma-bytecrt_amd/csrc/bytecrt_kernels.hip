@@ -387,6 +387,14 @@ __device__ __forceinline__ void encode_dispatch(const DevTables& t, const uint32
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t skew(uint32_t u) { return u + (u >> 4); }
 
+// Orders LDS accesses among the lanes of one wavefront (the LDS unit runs a
+// wave's instructions in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <uint32_t M>
 constexpr uint32_t exchange_stride() {
     constexpr uint32_t T = M / 16, base = M + M / 16;
@@ -421,7 +429,7 @@ __device__ __forceinline__ void fwd_cols_wg(const DevTables& t, uint32_t* __rest
         double* sc = s + cw * S;
 #pragma unroll
         for (int x = 0; x < 16; ++x) sc[skew(tau + T * x)] = v[x];
-        __syncthreads();
+        wave_lds_sync();
 #pragma unroll
         for (int x = 0; x < 16; ++x) v[x] = sc[skew(16 * tau + x)];
         ct_round<K2>(v, z, 1, 4, tau * (16 >> K2), p, pinv);
@@ -462,7 +470,7 @@ __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __rest
         double* sc = s + cw * S;
 #pragma unroll
         for (int x = 0; x < 16; ++x) sc[skew(16 * tau + x)] = v[x];
-        __syncthreads();
+        wave_lds_sync();
 #pragma unroll
         for (int x = 0; x < 16; ++x) v[x] = sc[skew(tau + T * x)];
     } else {
@@ -591,7 +599,7 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
         ra[skew(tau + 16 * x)] = uint32_t(balanced(v[x], p, pinv));
         rb[skew(tau + 16 * x)] = uint32_t(balanced(w[x], p, pinv));
     }
-    __syncthreads();
+    wave_lds_sync();
     // sub-levels 4..7 (positions 16 tau + x)
 #pragma unroll
     for (int x = 0; x < 16; ++x) {
@@ -608,7 +616,7 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
     __syncthreads();
 #pragma unroll
     for (int x = 0; x < 16; ++x) rx[skew(16 * tau + x)] = v[x];
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int x = 0; x < 16; ++x) v[x] = rx[skew(tau + 16 * x)];
     gs16_tw(v, twb, p, pinv);
@@ -638,23 +646,84 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
 // kernel.  Then, per pair of weight tiles (16 primes), the waves' residues go
 // into an FP64 exchange of 16 primes x CPW columns x N1 positions and each
 // group of N1 / 16 threads runs the column transform of one (prime, column)
-// exactly as k_fwd_cols does, storing balanced residues.
+// exactly as k_fwd_cols does, storing balanced residues.  Everything a pair
+// reads from global memory besides the data arrives ahead of use: its weight
+// tiles by direct global -> LDS loads during the previous pair's transform,
+// its primes' (p, 1/p) and first-round twiddles the same way during its MFMA
+// block, the second-round twiddles into registers during the first round.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kEfS = exchange_stride<256>();
+constexpr uint32_t kEfTw = 15;  // first-round twiddles per prime, z_j[1 .. 15]
 
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+// Dynamic LDS of the fused kernel: the staging area (later the exchange),
+// the per-prime dot-sum offsets, then one pair of weight tiles.  The current
+// pair's (p, 1/p) and first-round twiddles sit in 1,216 bytes of static LDS.
 size_t ef_smem_bytes(const DevTables& t) {
     const size_t stage = size_t(kEncCoefs) * (t.Kenc + 16), exch = size_t(16) * kEfS * sizeof(double);
     // + one pair of weight tiles (64 rows of Kenc + 16 bytes) when Kenc <= 224;
     // at Kenc = 256 that would leave room for only two workgroups per CU, so
     // that width reads its weight fragments from global memory instead
     const size_t wtiles = t.Kenc <= 224 && !(t.kflags & kFlagWeightsL2) ? size_t(64) * (t.Kenc + 16) : 0;
-    return std::max(stage, exch) + sizeof(EncPrime) * kMaxPrimesCrt + wtiles;
+    return std::max(stage, exch) + sizeof(double) * kMaxPrimesCrt + wtiles;
 }
 
-template <int KS, int NLOG1>
-__device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __restrict__ a,
-                                          const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
-                                          uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem) {
+// The twiddles a ct_round<K>(v, z, mult, e0, o0) uses, in its order of use
+// (loaded ahead of the round; (16 >> K) (2^K - 1) <= 15 of them).
+template <int K>
+__device__ __forceinline__ void load_round_tw(uint32_t (&tw)[15], const uint32_t* __restrict__ z, uint32_t mult,
+                                              uint32_t e0, uint32_t o0) {
+    int n = 0;
+#pragma unroll
+    for (int l = 0; l < K; ++l)
+#pragma unroll
+        for (int sb = 0; sb < (16 >> K); ++sb)
+#pragma unroll
+            for (int g = 0; g < (1 << l); ++g) tw[n++] = z[(mult << (e0 + l)) + ((o0 + sb) << l) + g];
+}
+
+template <int K>
+__device__ __forceinline__ void ct_round_pre(double (&v)[16], const uint32_t (&tw)[15], double p, double pinv) {
+    int n = 0;
+#pragma unroll
+    for (int l = 0; l < K; ++l) {
+        const int half = 1 << (K - 1 - l);
+#pragma unroll
+        for (int sb = 0; sb < (16 >> K); ++sb) {
+#pragma unroll
+            for (int g = 0; g < (1 << l); ++g) {
+                const double w = double(tw[n++]);
+#pragma unroll
+                for (int x = 0; x < half; ++x) {
+                    const int i0 = (sb << K) + g * 2 * half + x;
+                    ct_bfly(v[i0], v[i0 + half], w, p, pinv);
+                }
+            }
+        }
+    }
+}
+
+// One forward radix-16 round (column levels 0..3) with its 15 twiddles
+// tw[k] = z[k + 1] (in LDS).
+__device__ __forceinline__ void ct_round_tw15(double (&v)[16], const uint32_t* tw, double p, double pinv) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const int half = 8 >> l;
+#pragma unroll
+        for (int g = 0; g < (1 << l); ++g) {
+            const double w = double(tw[(1 << l) + g - 1]);
+#pragma unroll
+            for (int x = 0; x < half; ++x) ct_bfly(v[g * 2 * half + x], v[g * 2 * half + x + half], w, p, pinv);
+        }
+    }
+}
+
+template <int KS, int NLOG1, bool STAGE_W>
+__device__ __forceinline__ void encfwd_body(const DevTables& t, const uint32_t* __restrict__ a,
+                                            const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
+                                            uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem,
+                                            double* pbuf, uint32_t* twbuf) {
     static_assert(NLOG1 >= 6 && NLOG1 <= 8, "fused encode covers N1 = 64 .. 256");
     constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = 256 / N1, S = exchange_stride<N1>();
     constexpr int K2 = NLOG1 - 4;
@@ -662,28 +731,48 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
     static_assert(16 * CPW * S == 16 * kEfS, "exchange size");
     const uint32_t r0 = bx * CPW;
     const size_t stage = size_t(kEncCoefs) * (t.Kenc + 16), exch = size_t(16) * kEfS * sizeof(double);
-    EncPrime* cst = reinterpret_cast<EncPrime*>(smem + (stage > exch ? stage : exch));
+    double* offs = reinterpret_cast<double*>(smem + (stage > exch ? stage : exch));
+    uint8_t* wbuf = reinterpret_cast<uint8_t*>(offs + kMaxPrimesCrt);  // weight tiles of the current pair
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // Each pair's two weight tiles are copied into LDS once per workgroup (the
     // four waves then read their fragments there, instead of each loading
-    // them from L2), the next pair's during the current transform.
+    // them from L2), the next pair's during the current transform, by direct
+    // global -> LDS loads: no registers, and no wave waits for them before
+    // the barrier that ends the pair.  Slot k (16 bytes) of the tiles is
+    // byte 16 (k mod (2 KS + 1)) of row k / (2 KS + 1) (the last slot of a
+    // row is padding); wave instruction n fills slots 64 n .. 64 n + 63.
     // (kFlagWeightsL2 switches the staging off: every wave loads its fragments from L2)
-    const bool stage_w = KS <= 7 && !(t.kflags & kFlagWeightsL2);
-    uint8_t* wbuf = reinterpret_cast<uint8_t*>(cst + kMaxPrimesCrt);  // weight tiles of the current pair
-    const uint32_t wstride = t.Kenc + 16;
-    // copies weight tiles 2 grp, 2 grp + 1 (contiguous in wenc) into wbuf as 16-byte words
+    constexpr bool stage_w = STAGE_W;
     auto load_weights = [&](uint32_t grp) {
-        if (!stage_w) return;
-        const uint32_t rows = min(64u, (t.PTenc - 2 * grp) * 32), wpr = t.Kenc / 16;
-        const uint4* src = reinterpret_cast<const uint4*>(t.wenc + size_t(2 * grp) * 32 * t.Kenc);
-        for (uint32_t i = threadIdx.x; i < rows * wpr; i += kThreads) {
-            const uint32_t rw = i / wpr, cw = i - rw * wpr;
-            *reinterpret_cast<uint4*>(wbuf + rw * wstride + 16 * cw) = src[i];
+        if constexpr (!stage_w) return;
+        constexpr uint32_t SPR = 2 * KS + 1;
+        const uint32_t slots = min(64u, (t.PTenc - 2 * grp) * 32) * SPR;
+        const int8_t* src = t.wenc + size_t(2 * grp) * 32 * (32 * KS);
+        for (uint32_t n = wave; n * 64 < slots; n += 4) {
+            const uint32_t k = n * 64 + lane;
+            if (k < slots) {
+                const uint32_t rw = k / SPR, c16 = min(k - rw * SPR, 2 * KS - 1);
+                __builtin_amdgcn_global_load_lds(src + rw * (32 * KS) + 16 * c16, (lds_void_ptr)(wbuf + n * 1024),
+                                                 16, 0, 0);
+            }
+        }
+    };
+    // (p, 1/p) and z_j[1 .. 15] of the pair's primes j = 16 grp + jt (clamped
+    // to the last prime), the same way: lanes 0..15 of wave 0 take one 16-byte
+    // (p, 1/p) each, and wave w's lanes the twiddle words 64 w .. 64 w + 63.
+    auto load_pair_consts = [&](uint32_t grp) {
+        if (wave == 0 && lane < 16)
+            __builtin_amdgcn_global_load_lds(t.pd + 2 * min(16 * grp + lane, t.P - 1), (lds_void_ptr)pbuf, 16, 0, 0);
+        const uint32_t k = wave * 64 + lane;
+        if (k < 16 * kEfTw) {
+            const uint32_t jt = k / kEfTw, i = k - jt * kEfTw;
+            __builtin_amdgcn_global_load_lds(t.zf + size_t(min(16 * grp + jt, t.P - 1)) * t.N + 1 + i,
+                                             (lds_void_ptr)(twbuf + wave * 64), 4, 0, 0);
         }
     };
     for (uint32_t j = threadIdx.x; j < t.P; j += kThreads)
-        cst[j] = reinterpret_cast<const EncPrime*>(t.enc_prime)[j];
+        offs[j] = reinterpret_cast<const EncPrime*>(t.enc_prime)[j].off;
     const uint32_t* in = (op ? b : a) + size_t(prod) * t.N * t.L;
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t row = lane & 31, kh = lane >> 5;
     const uint32_t stride_bytes = t.Kenc + 16;
     // wave w holds coefficients s = (2 w + c) * 32 + row, c = 0, 1 (half w >> 1)
@@ -713,6 +802,7 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
     const uint32_t q = threadIdx.x / T, jt = q / CPW, cq = q % CPW, tau = threadIdx.x % T;
     const uint32_t npairs = (t.PTenc + 1) / 2;
     for (uint32_t grp = 0; grp < npairs; ++grp) {
+        load_pair_consts(grp);  // lands during the MFMA block (the barrier after it waits for it)
         // MFMA and VALU instructions share a SIMD's issue port: a wave in its
         // MFMA block goes first, so the matrix pipe stays fed while the other
         // resident workgroups' transforms fill the gaps (measured 1.5% faster)
@@ -722,10 +812,12 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
             const uint32_t pt = 2 * grp + u;
             if (pt < t.PTenc) {
                 v4i A[KS];
-                if (stage_w) {
-                    const uint8_t* wrow = wbuf + (u * 32 + row) * wstride + 16 * kh;
+                if constexpr (stage_w) {
+                    // an LDS-typed pointer: the two branches must not merge into generic (flat) loads
+                    typedef const __attribute__((address_space(3))) v4i lds_v4i;
+                    lds_v4i* wrow = (lds_v4i*)(wbuf + (u * 32 + row) * stride_bytes + 16 * kh);
 #pragma unroll
-                    for (int s2 = 0; s2 < KS; ++s2) A[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
+                    for (int s2 = 0; s2 < KS; ++s2) A[s2] = wrow[2 * s2];
                 } else {
                     const int8_t* wrow = t.wenc + (size_t(pt) * 32 + row) * t.Kenc + 16 * kh;
 #pragma unroll
@@ -733,7 +825,7 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
                 }
                 double off[4];
 #pragma unroll
-                for (int g = 0; g < 4; ++g) off[g] = cst[min(pt * 8 + 2 * g + kh, t.P - 1)].off;
+                for (int g = 0; g < 4; ++g) off[g] = offs[min(pt * 8 + 2 * g + kh, t.P - 1)];
                 v16i acc[2] = {v16i{}, v16i{}};
 #pragma unroll
                 for (int s2 = 0; s2 < KS; ++s2) {
@@ -756,7 +848,7 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
             }
         }
         __builtin_amdgcn_s_setprio(0);
-        __syncthreads();  // every wave is done with this pair's weights
+        __syncthreads();  // every wave is done with this pair's weights; its constants are in
         if (grp + 1 < npairs) load_weights(grp + 1);  // lands during the transform
         // forward column levels 0..n1-1 of prime j, column r0 + cq (k_fwd_cols)
         const uint32_t j = 16 * grp + jt;
@@ -764,20 +856,25 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
         const bool live = j < t.P;
         const uint32_t jc = live ? j : 0;
         const uint32_t* z = t.zf + size_t(jc) * t.N;
-        const double p = t.pd[2 * jc], pinv = t.pd[2 * jc + 1];
+        const double p = pbuf[2 * jt], pinv = pbuf[2 * jt + 1];
         double v[16];
+        uint32_t tw2[15];  // the second round's twiddles, in flight during the first (when registers allow)
         if (live) {
+            if constexpr (stage_w) load_round_tw<K2>(tw2, z, 1, 4, tau * (16 >> K2));
 #pragma unroll
             for (int x = 0; x < 16; ++x) v[x] = xc[skew(tau + T * x)];
-            ct_round<4>(v, z, 1, 0, 0, p, pinv);
+            ct_round_tw15(v, twbuf + jt * kEfTw, p, pinv);
 #pragma unroll
             for (int x = 0; x < 16; ++x) xc[skew(tau + T * x)] = v[x];
         }
-        __syncthreads();
+        wave_lds_sync();
         if (live) {
 #pragma unroll
             for (int x = 0; x < 16; ++x) v[x] = xc[skew(16 * tau + x)];
-            ct_round<K2>(v, z, 1, 4, tau * (16 >> K2), p, pinv);
+            if constexpr (stage_w)
+                ct_round_pre<K2>(v, tw2, p, pinv);
+            else
+                ct_round<K2>(v, z, 1, 4, tau * (16 >> K2), p, pinv);
             uint32_t* col = plane_ptr(t, planes, prod, op, j) + size_t(r0 + cq) * N1;
 #pragma unroll
             for (int x = 0; x < 16; x += 4)
@@ -785,8 +882,25 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
                     make_uint4(uint32_t(balanced(v[x], p, pinv)), uint32_t(balanced(v[x + 1], p, pinv)),
                                uint32_t(balanced(v[x + 2], p, pinv)), uint32_t(balanced(v[x + 3], p, pinv)));
         }
-        __syncthreads();  // the exchange is rewritten by the next pair
+        __syncthreads();  // the exchange, constants and weights are rewritten for the next pair
     }
+}
+
+// The weight staging (off at Kenc = 256 and under kFlagWeightsL2) is chosen
+// per launch; each choice is its own code path (a branch merged inside the
+// pair loop would make every wave wait for the staging loads of the other).
+template <int KS, int NLOG1>
+__device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __restrict__ a,
+                                          const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
+                                          uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem) {
+    // statically allocated, apart from the dynamic area, so the compiler sees
+    // that their loads do not alias the exchange and the weight tiles
+    __shared__ __align__(16) double pbuf[32];              // [16][2]: p, 1/p of the pair's primes
+    __shared__ __align__(16) uint32_t twbuf[16 * kEfTw];  // [16][15]: z_j[1 .. 15]
+    if (KS <= 7 && !(t.kflags & kFlagWeightsL2))
+        encfwd_body<KS, NLOG1, KS <= 7>(t, a, b, planes, bx, op, prod, smem, pbuf, twbuf);
+    else
+        encfwd_body<KS, NLOG1, false>(t, a, b, planes, bx, op, prod, smem, pbuf, twbuf);
 }
 
 // ---------------------------------------------------------------------------
