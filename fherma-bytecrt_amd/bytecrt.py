@@ -289,6 +289,7 @@ class Mix:
         self.schedule = (1, 1, 0)
 
     MERGE_CLASSES = 1
+    PERSISTENT = 2
 
     def run(self, stream=None, graph: bool = False):
         lib = load_library()
@@ -298,8 +299,8 @@ class Mix:
 
     def set_schedule(self, chunks: int = 1, streams: int = 1, flags: int = 0):
         """Chunks per geometry class, worker streams (1: all chunks on the
-        caller's stream) and flags (MERGE_CLASSES), each count in 1..16;
-        results are unchanged."""
+        caller's stream) and flags (MERGE_CLASSES, PERSISTENT), each count in
+        1..16; results are unchanged."""
         lib = load_library()
         rc = lib.bcrt_mix_set_schedule(self._h, chunks, streams, flags)
         if rc != BCRT_OK:

@@ -554,8 +554,9 @@ struct bcrt_mix {
     std::vector<bcrt_group> groups;  // as created
     uint32_t sched_chunks = 1;       // chunks per class
     uint32_t sched_streams = 1;      // worker streams (at most one per chunk)
-    uint32_t sched_flags = 0;        // BCRT_MIX_MERGE_CLASSES
+    uint32_t sched_flags = 0;        // BCRT_MIX_MERGE_CLASSES, BCRT_MIX_PERSISTENT
     std::vector<std::vector<bcrt::GroupDesc>> chunks;
+    uint32_t* queues = nullptr;      // BCRT_MIX_PERSISTENT: kQueueWords counters per chunk
     std::vector<bcrt_engine*> engines;  // referenced (mix_refs) for the mix's lifetime
     hipStream_t stream = nullptr;
     hipEvent_t graph_done = nullptr;  // recorded behind every graph launch
@@ -585,6 +586,8 @@ void drop_layout(bcrt_mix* m) {
     for (auto st : m->workers) (void)hipStreamDestroy(st);
     m->workers.clear();
     m->chunks.clear();
+    if (m->queues) (void)hipFree(m->queues);
+    m->queues = nullptr;
 }
 
 // drop_layout, then the mix's own stream and event.
@@ -646,6 +649,9 @@ void layout_mix(bcrt_mix* m) {
     }
     const uint32_t nchunks = uint32_t(m->chunks.size());
     const uint32_t nstreams = std::min(m->sched_streams, nchunks);
+    if (m->sched_flags & BCRT_MIX_PERSISTENT)
+        check(hipMalloc(reinterpret_cast<void**>(&m->queues), sizeof(uint32_t) * bcrt::kQueueWords * nchunks),
+              "allocate mix work queues");
     if (nstreams > 1) {
         for (uint32_t c = 0; c < nstreams; ++c) {
             hipStream_t st = nullptr;
@@ -661,9 +667,11 @@ void layout_mix(bcrt_mix* m) {
 // events of `ev` when the schedule has several (also valid inside a stream
 // capture).
 void enqueue_mix(bcrt_mix* m, hipStream_t origin, EventSet& ev) {
+    auto queue = [m](size_t c) { return m->queues ? m->queues + c * bcrt::kQueueWords : nullptr; };
     if (m->workers.empty()) {
-        for (const auto& chunk : m->chunks)
-            check(bcrt::launch_product_groups(chunk.data(), uint32_t(chunk.size()), origin), "launch mixed batch");
+        for (size_t c = 0; c < m->chunks.size(); ++c)
+            check(bcrt::launch_product_groups(m->chunks[c].data(), uint32_t(m->chunks[c].size()), origin, queue(c)),
+                  "launch mixed batch");
         return;
     }
     const size_t ns = m->workers.size();
@@ -671,7 +679,8 @@ void enqueue_mix(bcrt_mix* m, hipStream_t origin, EventSet& ev) {
     check(hipEventRecord(ev.fork, origin), "fork");
     for (size_t w = 0; w < ns; ++w) check(hipStreamWaitEvent(m->workers[w], ev.fork, 0), "fork wait");
     for (size_t c = 0; c < m->chunks.size(); ++c)
-        check(bcrt::launch_product_groups(m->chunks[c].data(), uint32_t(m->chunks[c].size()), m->workers[c % ns]),
+        check(bcrt::launch_product_groups(m->chunks[c].data(), uint32_t(m->chunks[c].size()), m->workers[c % ns],
+                                          queue(c)),
               "launch mixed chunk");
     for (size_t w = 0; w < ns; ++w) {
         check(hipEventRecord(ev.joins[w], m->workers[w]), "join record");
@@ -777,9 +786,9 @@ extern "C" const char* bcrt_mix_last_error(const bcrt_mix* m) { return m ? m->la
 extern "C" int bcrt_mix_set_schedule(bcrt_mix* m, uint32_t chunks, uint32_t streams, uint32_t flags) {
     if (!m) return BCRT_EINVAL;
     if (chunks < 1 || chunks > kMixMaxChunks || streams < 1 || streams > kMixMaxChunks ||
-        (flags & ~uint32_t(BCRT_MIX_MERGE_CLASSES))) {
+        (flags & ~uint32_t(BCRT_MIX_MERGE_CLASSES | BCRT_MIX_PERSISTENT))) {
         m->last_error = "ByteCRT mix schedule needs 1 <= chunks, streams <= " + std::to_string(kMixMaxChunks) +
-                        " and flags within BCRT_MIX_MERGE_CLASSES";
+                        " and flags within BCRT_MIX_MERGE_CLASSES | BCRT_MIX_PERSISTENT";
         return BCRT_EINVAL;
     }
     try {

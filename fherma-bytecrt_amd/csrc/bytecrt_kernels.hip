@@ -25,6 +25,9 @@
 #include "bytecrt_kernels.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace bcrt {
 namespace {
@@ -1288,21 +1291,21 @@ __global__ __launch_bounds__(kCrtThreads, 5) void k_crt(DevTables t, const uint3
     crt_tiles<KS>(t, planes, out, x, z, smem_crt);
 }
 
-// Finds this workgroup's group and its (x, y, z) in that group's grid, or
-// returns null for the padding workgroups.  Each group's workgroups start at a
-// multiple of 8 (group_workgroups pads the count), so the XCD renumbering is
-// applied inside every group: each XCD gets an eighth of every group's tiles,
-// whatever their cost.
+// Finds the group of workgroup slot `id` of a mixed launch and the slot's
+// (x, y, z) in that group's grid, or returns false for the padding slots.
+// Each group's slots start at a multiple of 8 (group_workgroups pads the
+// count), so the XCD renumbering is applied inside every group: each XCD gets
+// an eighth of every group's tiles, whatever their cost.
 // (The table is only ever indexed, never addressed, so it stays in the kernel
 // argument segment.)
-__device__ __forceinline__ bool locate(const GroupTable& g, int stage, uint32_t* k_out, uint32_t* x, uint32_t* y,
-                                       uint32_t* z) {
+__device__ __forceinline__ bool locate(const GroupTable& g, int stage, uint32_t id, uint32_t* k_out, uint32_t* x,
+                                       uint32_t* y, uint32_t* z) {
     uint32_t k = 0;
-    while (k + 1 < g.n && blockIdx.x >= g.g[k + 1].begin[stage]) ++k;
+    while (k + 1 < g.n && id >= g.g[k + 1].begin[stage]) ++k;
     uint32_t gx, gy, gz;
     stage_grid(g.g[k].t, g.g[k].args.batch, stage, &gx, &gy, &gz);
     const uint32_t count = gx * gy * gz;
-    const uint32_t tile = xcd_tile(blockIdx.x - g.g[k].begin[stage], (count + 7) / 8 * 8, g.g[k].t.kflags);
+    const uint32_t tile = xcd_tile(id - g.g[k].begin[stage], (count + 7) / 8 * 8, g.g[k].t.kflags);
     *k_out = __builtin_amdgcn_readfirstlane(k);  // uniform: scalar registers
     tile_xyz(tile, gx, gy, x, y, z);
     *x = __builtin_amdgcn_readfirstlane(*x);
@@ -1311,93 +1314,123 @@ __device__ __forceinline__ bool locate(const GroupTable& g, int stage, uint32_t*
     return tile < count;
 }
 
+// Runs body(d, x, y, z) for this workgroup's tile(s) of stage `stage`, d
+// being a copy of the tile's group descriptor (a copy: its fields are not
+// loaded again after every barrier): the slot blockIdx.x, or, with a work
+// queue, slots pulled one at a time from the counter of this workgroup's XCD
+// (workgroup b runs on XCD b mod 8; slot ids 8 n + x go to XCD x, which keeps
+// the renumbered tiles of each XCD on it) until the stage's slots run out.
+// Every counter only grows and every workgroup stops at the first slot past
+// the end, so the grid always drains.  (The body captures nothing: a lambda
+// holding a reference to the by-value table would copy it to scratch.)  The
+// two modes are separate kernels (PQ): one function holding both copies of a
+// body gets the registers of the larger.
+template <bool PQ, class Body>
+__device__ __forceinline__ void for_each_tile(const GroupTable& g, int stage, Body body) {
+    uint32_t k, x, y, z;
+    if constexpr (!PQ) {
+        if (locate(g, stage, blockIdx.x, &k, &x, &y, &z)) {
+            const GroupDesc d = g.g[k];
+            body(d, x, y, z);
+        }
+    } else {
+        __shared__ uint32_t next_slot;
+        const uint32_t xcd = blockIdx.x & 7;
+        uint32_t* ctr = g.queue + 8 * stage + xcd;
+        for (;;) {
+            if (threadIdx.x == 0) next_slot = atomicAdd(ctr, 1u);
+            __syncthreads();
+            const uint32_t id = __builtin_amdgcn_readfirstlane(next_slot) * 8 + xcd;
+            if (id >= g.total[stage]) return;  // uniform: every wave read the same slot
+            if (locate(g, stage, id, &k, &x, &y, &z)) {
+                const GroupDesc d = g.g[k];
+                body(d, x, y, z);
+            }
+            __syncthreads();  // the slot word and the body's LDS are free for the next tile
+        }
+    }
+}
+
+__global__ void k_queue_reset(uint32_t* __restrict__ queue) {
+    if (threadIdx.x < kQueueWords) queue[threadIdx.x] = 0;
+}
+
 // KS / NLOG1 = 0: any mix of shapes (switch per workgroup); otherwise every
 // group of the launch has that shape and the code is specialised for it.
-template <int KS>
+template <int KS, bool PQ>
 __global__ __launch_bounds__(kThreads, 3) void k_encode_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_enc_g[];
-    uint32_t x, y, z;
-    uint32_t k;
-    if (!locate(g, 0, &k, &x, &y, &z)) return;
-    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
-    if constexpr (KS == 0)
-        encode_dispatch(d.t, d.args.a, d.args.b, d.args.planes, x, y, z, smem_enc_g);
-    else
-        encode_wg<KS>(d.t, d.args.a, d.args.b, d.args.planes, x, y, z, smem_enc_g);
+    for_each_tile<PQ>(g, 0, [](const GroupDesc& d, uint32_t x, uint32_t y, uint32_t z) {
+        if constexpr (KS == 0)
+            encode_dispatch(d.t, d.args.a, d.args.b, d.args.planes, x, y, z, smem_enc_g);
+        else
+            encode_wg<KS>(d.t, d.args.a, d.args.b, d.args.planes, x, y, z, smem_enc_g);
+    });
 }
 
-template <int KS, int NLOG1>
+template <int KS, int NLOG1, bool PQ>
 __global__ __launch_bounds__(kThreads, 3) void k_encode_fwd_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_ef_g[];
-    uint32_t r, op, prod, k;
-    if (!locate(g, 5, &k, &r, &op, &prod)) return;
-    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
-    encfwd_wg<KS, NLOG1>(d.t, d.args.a, d.args.b, d.args.planes, __builtin_amdgcn_readfirstlane(r),
-                  __builtin_amdgcn_readfirstlane(op), __builtin_amdgcn_readfirstlane(prod), smem_ef_g);
+    for_each_tile<PQ>(g, 5, [](const GroupDesc& d, uint32_t r, uint32_t op, uint32_t prod) {
+        encfwd_wg<KS, NLOG1>(d.t, d.args.a, d.args.b, d.args.planes, r, op, prod, smem_ef_g);
+    });
 }
 
-template <int NLOG1>
+template <int NLOG1, bool PQ>
 __global__ __launch_bounds__(kThreads) void k_fwd_cols_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_fc_g[];
-    uint32_t x, y, z;
-    uint32_t k;
-    if (!locate(g, 1, &k, &x, &y, &z)) return;
-    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
-    double* s = reinterpret_cast<double*>(smem_fc_g);
-    if constexpr (NLOG1 != 0) {
-        fwd_cols_wg<NLOG1>(d.t, d.args.planes, x, y, z, s);
-    } else {
-        switch (d.t.n1) {
-            case 4: fwd_cols_wg<4>(d.t, d.args.planes, x, y, z, s); break;
-            case 5: fwd_cols_wg<5>(d.t, d.args.planes, x, y, z, s); break;
-            case 6: fwd_cols_wg<6>(d.t, d.args.planes, x, y, z, s); break;
-            case 7: fwd_cols_wg<7>(d.t, d.args.planes, x, y, z, s); break;
-            default: fwd_cols_wg<8>(d.t, d.args.planes, x, y, z, s); break;
+    for_each_tile<PQ>(g, 1, [](const GroupDesc& d, uint32_t x, uint32_t y, uint32_t z) {
+        double* s = reinterpret_cast<double*>(smem_fc_g);
+        if constexpr (NLOG1 != 0) {
+            fwd_cols_wg<NLOG1>(d.t, d.args.planes, x, y, z, s);
+        } else {
+            switch (d.t.n1) {
+                case 4: fwd_cols_wg<4>(d.t, d.args.planes, x, y, z, s); break;
+                case 5: fwd_cols_wg<5>(d.t, d.args.planes, x, y, z, s); break;
+                case 6: fwd_cols_wg<6>(d.t, d.args.planes, x, y, z, s); break;
+                case 7: fwd_cols_wg<7>(d.t, d.args.planes, x, y, z, s); break;
+                default: fwd_cols_wg<8>(d.t, d.args.planes, x, y, z, s); break;
+            }
         }
-    }
+    });
 }
 
+template <bool PQ>
 __global__ __launch_bounds__(kThreads) void k_middle_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_mid_g[];
-    uint32_t x, y, z;
-    uint32_t k;
-    if (!locate(g, 2, &k, &x, &y, &z)) return;
-    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
-    middle_wg(d.t, d.args.planes, x, y, z, reinterpret_cast<double*>(smem_mid_g));
+    for_each_tile<PQ>(g, 2, [](const GroupDesc& d, uint32_t x, uint32_t y, uint32_t z) {
+        middle_wg(d.t, d.args.planes, x, y, z, reinterpret_cast<double*>(smem_mid_g));
+    });
 }
 
-template <int NLOG1>
+template <int NLOG1, bool PQ>
 __global__ __launch_bounds__(kThreads) void k_inv_cols_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_ic_g[];
-    uint32_t x, y, z;
-    uint32_t k;
-    if (!locate(g, 3, &k, &x, &y, &z)) return;
-    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
-    double* s = reinterpret_cast<double*>(smem_ic_g);
-    if constexpr (NLOG1 != 0) {
-        inv_cols_wg<NLOG1>(d.t, d.args.planes, x, z, s);
-    } else {
-        switch (d.t.n1) {
-            case 4: inv_cols_wg<4>(d.t, d.args.planes, x, z, s); break;
-            case 5: inv_cols_wg<5>(d.t, d.args.planes, x, z, s); break;
-            case 6: inv_cols_wg<6>(d.t, d.args.planes, x, z, s); break;
-            case 7: inv_cols_wg<7>(d.t, d.args.planes, x, z, s); break;
-            default: inv_cols_wg<8>(d.t, d.args.planes, x, z, s); break;
+    for_each_tile<PQ>(g, 3, [](const GroupDesc& d, uint32_t x, uint32_t, uint32_t z) {
+        double* s = reinterpret_cast<double*>(smem_ic_g);
+        if constexpr (NLOG1 != 0) {
+            inv_cols_wg<NLOG1>(d.t, d.args.planes, x, z, s);
+        } else {
+            switch (d.t.n1) {
+                case 4: inv_cols_wg<4>(d.t, d.args.planes, x, z, s); break;
+                case 5: inv_cols_wg<5>(d.t, d.args.planes, x, z, s); break;
+                case 6: inv_cols_wg<6>(d.t, d.args.planes, x, z, s); break;
+                case 7: inv_cols_wg<7>(d.t, d.args.planes, x, z, s); break;
+                default: inv_cols_wg<8>(d.t, d.args.planes, x, z, s); break;
+            }
         }
-    }
+    });
 }
 
-template <int KS>
+template <int KS, bool PQ>
 __global__ __launch_bounds__(kCrtThreads, 5) void k_crt_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_crt_g[];
-    uint32_t x, y, z;
-    uint32_t k;
-    if (!locate(g, 4, &k, &x, &y, &z)) return;
-    const GroupDesc d = g.g[k];  // copied: field loads are not repeated after barriers
-    if constexpr (KS == 0)
-        crt_dispatch(d.t, d.args.planes, d.args.out, x, z, smem_crt_g);
-    else
-        crt_tiles<KS>(d.t, d.args.planes, d.args.out, x, z, smem_crt_g);
+    for_each_tile<PQ>(g, 4, [](const GroupDesc& d, uint32_t x, uint32_t, uint32_t z) {
+        if constexpr (KS == 0)
+            crt_dispatch(d.t, d.args.planes, d.args.out, x, z, smem_crt_g);
+        else
+            crt_tiles<KS>(d.t, d.args.planes, d.args.out, x, z, smem_crt_g);
+    });
 }
 
 bool shapes_ok(const DevTables& t, const BatchArgs& args) {
@@ -1536,25 +1569,133 @@ uint32_t group_workgroups(const DevTables& t, uint32_t batch, int stage) {
     return (x * y * z + 7) / 8 * 8;  // padded to whole rounds of the 8 XCDs
 }
 
-template <int NLOG1>
-void launch_encfwd_groups(uint32_t ks, uint32_t tiles, size_t sm, const GroupTable& table, hipStream_t stream) {
-    const dim3 g(tiles), blk(kThreads);
+// Workgroups of `kern` (`threads` each, `sm` bytes of dynamic LDS) the
+// current device holds at once, rounded down to whole rounds of the 8 XCDs.
+uint32_t resident_grid(const void* kern, uint32_t threads, size_t sm) {
+    static std::mutex m;
+    static std::map<std::tuple<int, const void*, uint32_t, size_t>, uint32_t> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 8;
+    const auto key = std::make_tuple(dev, kern, threads, sm);
+    std::lock_guard<std::mutex> g(m);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, int(threads), sm) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        per_cu = cus = 1;
+    const uint32_t n = std::max<uint32_t>(8, uint32_t(std::max(per_cu, 1) * std::max(cus, 1)) / 8 * 8);
+    cache.emplace(key, n);
+    return n;
+}
+
+// Launches group kernel `kern` over `tiles` workgroup slots: one workgroup per
+// slot, or, when the table has a work queue, a resident grid whose
+// workgroups pull the slots.
+void launch_g(const void* kern, uint32_t tiles, uint32_t threads, size_t sm, GroupTable& table,
+              hipStream_t stream) {
+    const uint32_t grid = table.queue ? std::min(tiles, resident_grid(kern, threads, sm)) : tiles;
+    void* args[] = {&table};
+    (void)hipLaunchKernel(kern, dim3(grid), dim3(threads), args, sm, stream);
+}
+
+template <int NLOG1, bool PQ>
+void launch_encfwd_groups(uint32_t ks, uint32_t tiles, size_t sm, GroupTable& table, hipStream_t stream) {
+    const void* k = nullptr;
     switch (ks) {
-        case 1: hipLaunchKernelGGL((k_encode_fwd_groups<1, NLOG1>), g, blk, sm, stream, table); break;
-        case 2: hipLaunchKernelGGL((k_encode_fwd_groups<2, NLOG1>), g, blk, sm, stream, table); break;
-        case 3: hipLaunchKernelGGL((k_encode_fwd_groups<3, NLOG1>), g, blk, sm, stream, table); break;
-        case 4: hipLaunchKernelGGL((k_encode_fwd_groups<4, NLOG1>), g, blk, sm, stream, table); break;
-        case 5: hipLaunchKernelGGL((k_encode_fwd_groups<5, NLOG1>), g, blk, sm, stream, table); break;
-        case 6: hipLaunchKernelGGL((k_encode_fwd_groups<6, NLOG1>), g, blk, sm, stream, table); break;
-        case 7: hipLaunchKernelGGL((k_encode_fwd_groups<7, NLOG1>), g, blk, sm, stream, table); break;
-        default: hipLaunchKernelGGL((k_encode_fwd_groups<8, NLOG1>), g, blk, sm, stream, table); break;
+        case 1: k = reinterpret_cast<const void*>(&k_encode_fwd_groups<1, NLOG1, PQ>); break;
+        case 2: k = reinterpret_cast<const void*>(&k_encode_fwd_groups<2, NLOG1, PQ>); break;
+        case 3: k = reinterpret_cast<const void*>(&k_encode_fwd_groups<3, NLOG1, PQ>); break;
+        case 4: k = reinterpret_cast<const void*>(&k_encode_fwd_groups<4, NLOG1, PQ>); break;
+        case 5: k = reinterpret_cast<const void*>(&k_encode_fwd_groups<5, NLOG1, PQ>); break;
+        case 6: k = reinterpret_cast<const void*>(&k_encode_fwd_groups<6, NLOG1, PQ>); break;
+        case 7: k = reinterpret_cast<const void*>(&k_encode_fwd_groups<7, NLOG1, PQ>); break;
+        default: k = reinterpret_cast<const void*>(&k_encode_fwd_groups<8, NLOG1, PQ>); break;
+    }
+    launch_g(k, tiles, kThreads, sm, table, stream);
+}
+
+// The stage launches of launch_product_groups, with the one-tile-per-workgroup
+// (PQ = false) or the work-queue (PQ = true) kernels.
+template <bool PQ>
+void enqueue_groups(const GroupDesc* groups, uint32_t ngroups, GroupTable& table,
+                    const uint32_t (&total)[kGroupStages], size_t enc, size_t crt, size_t ef, hipStream_t stream) {
+    auto go = [&](const void* kern, uint32_t tiles, uint32_t threads, size_t sm) {
+        launch_g(kern, tiles, threads, sm, table, stream);
+    };
+    // Specialised kernels when every group shares the shape they depend on.
+    bool same_k = true, same_n1 = true, same_kc = true;
+    for (uint32_t k = 1; k < ngroups; ++k) {
+        same_k = same_k && groups[k].t.Kenc == groups[0].t.Kenc;
+        same_n1 = same_n1 && groups[k].t.n1 == groups[0].t.n1;
+        same_kc = same_kc && groups[k].t.Kcrt == groups[0].t.Kcrt;
+    }
+    const uint32_t ks = same_k ? groups[0].t.Kenc / 32 : 0, n1 = same_n1 ? groups[0].t.n1 : 0;
+    const size_t xb = kExchangeBytes;
+    if (ks != 0 && n1 >= 6 && total[5] >= groups[0].t.fuse_min_tiles) {
+        // every group has the same Kenc and N1 >= 64: encode and forward columns in one kernel
+        switch (n1) {
+            case 6: launch_encfwd_groups<6, PQ>(ks, total[5], ef, table, stream); break;
+            case 7: launch_encfwd_groups<7, PQ>(ks, total[5], ef, table, stream); break;
+            default: launch_encfwd_groups<8, PQ>(ks, total[5], ef, table, stream); break;
+        }
+    } else {
+        switch (ks) {
+            case 1: go(reinterpret_cast<const void*>(&k_encode_groups<1, PQ>), total[0], kThreads, enc); break;
+            case 2: go(reinterpret_cast<const void*>(&k_encode_groups<2, PQ>), total[0], kThreads, enc); break;
+            case 3: go(reinterpret_cast<const void*>(&k_encode_groups<3, PQ>), total[0], kThreads, enc); break;
+            case 4: go(reinterpret_cast<const void*>(&k_encode_groups<4, PQ>), total[0], kThreads, enc); break;
+            case 5: go(reinterpret_cast<const void*>(&k_encode_groups<5, PQ>), total[0], kThreads, enc); break;
+            case 6: go(reinterpret_cast<const void*>(&k_encode_groups<6, PQ>), total[0], kThreads, enc); break;
+            case 7: go(reinterpret_cast<const void*>(&k_encode_groups<7, PQ>), total[0], kThreads, enc); break;
+            case 8: go(reinterpret_cast<const void*>(&k_encode_groups<8, PQ>), total[0], kThreads, enc); break;
+            default: go(reinterpret_cast<const void*>(&k_encode_groups<0, PQ>), total[0], kThreads, enc); break;
+        }
+        switch (n1) {
+            case 4: go(reinterpret_cast<const void*>(&k_fwd_cols_groups<4, PQ>), total[1], kThreads, xb); break;
+            case 5: go(reinterpret_cast<const void*>(&k_fwd_cols_groups<5, PQ>), total[1], kThreads, xb); break;
+            case 6: go(reinterpret_cast<const void*>(&k_fwd_cols_groups<6, PQ>), total[1], kThreads, xb); break;
+            case 7: go(reinterpret_cast<const void*>(&k_fwd_cols_groups<7, PQ>), total[1], kThreads, xb); break;
+            case 8: go(reinterpret_cast<const void*>(&k_fwd_cols_groups<8, PQ>), total[1], kThreads, xb); break;
+            default: go(reinterpret_cast<const void*>(&k_fwd_cols_groups<0, PQ>), total[1], kThreads, xb); break;
+        }
+    }
+    go(reinterpret_cast<const void*>(&k_middle_groups<PQ>), total[2], kThreads, xb);
+    switch (n1) {
+        case 4: go(reinterpret_cast<const void*>(&k_inv_cols_groups<4, PQ>), total[3], kThreads, xb); break;
+        case 5: go(reinterpret_cast<const void*>(&k_inv_cols_groups<5, PQ>), total[3], kThreads, xb); break;
+        case 6: go(reinterpret_cast<const void*>(&k_inv_cols_groups<6, PQ>), total[3], kThreads, xb); break;
+        case 7: go(reinterpret_cast<const void*>(&k_inv_cols_groups<7, PQ>), total[3], kThreads, xb); break;
+        case 8: go(reinterpret_cast<const void*>(&k_inv_cols_groups<8, PQ>), total[3], kThreads, xb); break;
+        default: go(reinterpret_cast<const void*>(&k_inv_cols_groups<0, PQ>), total[3], kThreads, xb); break;
+    }
+    switch (same_kc ? groups[0].t.Kcrt / 32 : 0) {
+        case 1: go(reinterpret_cast<const void*>(&k_crt_groups<1, PQ>), total[4], kCrtThreads, crt); break;
+        case 2: go(reinterpret_cast<const void*>(&k_crt_groups<2, PQ>), total[4], kCrtThreads, crt); break;
+        case 3: go(reinterpret_cast<const void*>(&k_crt_groups<3, PQ>), total[4], kCrtThreads, crt); break;
+        case 4: go(reinterpret_cast<const void*>(&k_crt_groups<4, PQ>), total[4], kCrtThreads, crt); break;
+        case 5: go(reinterpret_cast<const void*>(&k_crt_groups<5, PQ>), total[4], kCrtThreads, crt); break;
+        case 6: go(reinterpret_cast<const void*>(&k_crt_groups<6, PQ>), total[4], kCrtThreads, crt); break;
+        case 7: go(reinterpret_cast<const void*>(&k_crt_groups<7, PQ>), total[4], kCrtThreads, crt); break;
+        case 8: go(reinterpret_cast<const void*>(&k_crt_groups<8, PQ>), total[4], kCrtThreads, crt); break;
+        case 9: go(reinterpret_cast<const void*>(&k_crt_groups<9, PQ>), total[4], kCrtThreads, crt); break;
+        case 10: go(reinterpret_cast<const void*>(&k_crt_groups<10, PQ>), total[4], kCrtThreads, crt); break;
+        case 11: go(reinterpret_cast<const void*>(&k_crt_groups<11, PQ>), total[4], kCrtThreads, crt); break;
+        case 12: go(reinterpret_cast<const void*>(&k_crt_groups<12, PQ>), total[4], kCrtThreads, crt); break;
+        case 13: go(reinterpret_cast<const void*>(&k_crt_groups<13, PQ>), total[4], kCrtThreads, crt); break;
+        case 14: go(reinterpret_cast<const void*>(&k_crt_groups<14, PQ>), total[4], kCrtThreads, crt); break;
+        case 15: go(reinterpret_cast<const void*>(&k_crt_groups<15, PQ>), total[4], kCrtThreads, crt); break;
+        case 16: go(reinterpret_cast<const void*>(&k_crt_groups<16, PQ>), total[4], kCrtThreads, crt); break;
+        case 17: go(reinterpret_cast<const void*>(&k_crt_groups<17, PQ>), total[4], kCrtThreads, crt); break;
+        default: go(reinterpret_cast<const void*>(&k_crt_groups<0, PQ>), total[4], kCrtThreads, crt); break;
     }
 }
 
-hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipStream_t stream) {
+hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipStream_t stream, uint32_t* queue) {
     if (ngroups == 0 || ngroups > kMaxGroups) return hipErrorInvalidValue;
     GroupTable table{};
     table.n = ngroups;
+    table.queue = queue;
     size_t enc = 0, crt = 0, ef = 0;
     uint32_t total[kGroupStages] = {};
     for (uint32_t k = 0; k < ngroups; ++k) {
@@ -1569,83 +1710,23 @@ hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipS
         }
         table.g[k] = g;
     }
-    // Specialised kernels when every group shares the shape they depend on.
-    bool same_k = true, same_n1 = true, same_kc = true;
-    for (uint32_t k = 1; k < ngroups; ++k) {
-        same_k = same_k && groups[k].t.Kenc == groups[0].t.Kenc;
-        same_n1 = same_n1 && groups[k].t.n1 == groups[0].t.n1;
-        same_kc = same_kc && groups[k].t.Kcrt == groups[0].t.Kcrt;
-    }
-    const uint32_t ks = same_k ? groups[0].t.Kenc / 32 : 0, n1 = same_n1 ? groups[0].t.n1 : 0;
-    const dim3 blk(kThreads);
-    const size_t xb = kExchangeBytes;
-    if (ks != 0 && n1 >= 6 && total[5] >= groups[0].t.fuse_min_tiles) {
-        // every group has the same Kenc and N1 >= 64: encode and forward columns in one kernel
-        switch (n1) {
-            case 6: launch_encfwd_groups<6>(ks, total[5], ef, table, stream); break;
-            case 7: launch_encfwd_groups<7>(ks, total[5], ef, table, stream); break;
-            default: launch_encfwd_groups<8>(ks, total[5], ef, table, stream); break;
-        }
+    for (int st = 0; st < kGroupStages; ++st) table.total[st] = total[st];
+    if (queue) {
+        hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, stream, queue);  // a kernel, not a memset node
+        enqueue_groups<true>(groups, ngroups, table, total, enc, crt, ef, stream);
     } else {
-        switch (ks) {
-            case 1: hipLaunchKernelGGL(k_encode_groups<1>, dim3(total[0]), blk, enc, stream, table); break;
-            case 2: hipLaunchKernelGGL(k_encode_groups<2>, dim3(total[0]), blk, enc, stream, table); break;
-            case 3: hipLaunchKernelGGL(k_encode_groups<3>, dim3(total[0]), blk, enc, stream, table); break;
-            case 4: hipLaunchKernelGGL(k_encode_groups<4>, dim3(total[0]), blk, enc, stream, table); break;
-            case 5: hipLaunchKernelGGL(k_encode_groups<5>, dim3(total[0]), blk, enc, stream, table); break;
-            case 6: hipLaunchKernelGGL(k_encode_groups<6>, dim3(total[0]), blk, enc, stream, table); break;
-            case 7: hipLaunchKernelGGL(k_encode_groups<7>, dim3(total[0]), blk, enc, stream, table); break;
-            case 8: hipLaunchKernelGGL(k_encode_groups<8>, dim3(total[0]), blk, enc, stream, table); break;
-            default: hipLaunchKernelGGL(k_encode_groups<0>, dim3(total[0]), blk, enc, stream, table); break;
-        }
-        switch (n1) {
-            case 4: hipLaunchKernelGGL(k_fwd_cols_groups<4>, dim3(total[1]), blk, xb, stream, table); break;
-            case 5: hipLaunchKernelGGL(k_fwd_cols_groups<5>, dim3(total[1]), blk, xb, stream, table); break;
-            case 6: hipLaunchKernelGGL(k_fwd_cols_groups<6>, dim3(total[1]), blk, xb, stream, table); break;
-            case 7: hipLaunchKernelGGL(k_fwd_cols_groups<7>, dim3(total[1]), blk, xb, stream, table); break;
-            case 8: hipLaunchKernelGGL(k_fwd_cols_groups<8>, dim3(total[1]), blk, xb, stream, table); break;
-            default: hipLaunchKernelGGL(k_fwd_cols_groups<0>, dim3(total[1]), blk, xb, stream, table); break;
-        }
-    }
-    hipLaunchKernelGGL(k_middle_groups, dim3(total[2]), blk, xb, stream, table);
-    switch (n1) {
-        case 4: hipLaunchKernelGGL(k_inv_cols_groups<4>, dim3(total[3]), blk, xb, stream, table); break;
-        case 5: hipLaunchKernelGGL(k_inv_cols_groups<5>, dim3(total[3]), blk, xb, stream, table); break;
-        case 6: hipLaunchKernelGGL(k_inv_cols_groups<6>, dim3(total[3]), blk, xb, stream, table); break;
-        case 7: hipLaunchKernelGGL(k_inv_cols_groups<7>, dim3(total[3]), blk, xb, stream, table); break;
-        case 8: hipLaunchKernelGGL(k_inv_cols_groups<8>, dim3(total[3]), blk, xb, stream, table); break;
-        default: hipLaunchKernelGGL(k_inv_cols_groups<0>, dim3(total[3]), blk, xb, stream, table); break;
-    }
-    const dim3 gc(total[4]), bc(kCrtThreads);
-    switch (same_kc ? groups[0].t.Kcrt / 32 : 0) {
-        case 1: hipLaunchKernelGGL(k_crt_groups<1>, gc, bc, crt, stream, table); break;
-        case 2: hipLaunchKernelGGL(k_crt_groups<2>, gc, bc, crt, stream, table); break;
-        case 3: hipLaunchKernelGGL(k_crt_groups<3>, gc, bc, crt, stream, table); break;
-        case 4: hipLaunchKernelGGL(k_crt_groups<4>, gc, bc, crt, stream, table); break;
-        case 5: hipLaunchKernelGGL(k_crt_groups<5>, gc, bc, crt, stream, table); break;
-        case 6: hipLaunchKernelGGL(k_crt_groups<6>, gc, bc, crt, stream, table); break;
-        case 7: hipLaunchKernelGGL(k_crt_groups<7>, gc, bc, crt, stream, table); break;
-        case 8: hipLaunchKernelGGL(k_crt_groups<8>, gc, bc, crt, stream, table); break;
-        case 9: hipLaunchKernelGGL(k_crt_groups<9>, gc, bc, crt, stream, table); break;
-        case 10: hipLaunchKernelGGL(k_crt_groups<10>, gc, bc, crt, stream, table); break;
-        case 11: hipLaunchKernelGGL(k_crt_groups<11>, gc, bc, crt, stream, table); break;
-        case 12: hipLaunchKernelGGL(k_crt_groups<12>, gc, bc, crt, stream, table); break;
-        case 13: hipLaunchKernelGGL(k_crt_groups<13>, gc, bc, crt, stream, table); break;
-        case 14: hipLaunchKernelGGL(k_crt_groups<14>, gc, bc, crt, stream, table); break;
-        case 15: hipLaunchKernelGGL(k_crt_groups<15>, gc, bc, crt, stream, table); break;
-        case 16: hipLaunchKernelGGL(k_crt_groups<16>, gc, bc, crt, stream, table); break;
-        case 17: hipLaunchKernelGGL(k_crt_groups<17>, gc, bc, crt, stream, table); break;
-        default: hipLaunchKernelGGL(k_crt_groups<0>, gc, bc, crt, stream, table); break;
+        enqueue_groups<false>(groups, ngroups, table, total, enc, crt, ef, stream);
     }
     return hipGetLastError();
 }
 
 hipError_t configure_kernels() {
     // Every kernel's dynamic LDS stays within the 64 KiB default (the largest,
-    // the fused encode at Kenc = 224 with its staged weight tiles, takes 52 KB;
-    // the CRT at most 35.8 KB), so no attribute needs raising.
+    // the fused encode at Kenc = 224 with its staged weight tiles, takes 50 KB
+    // besides 1.2 KB of static LDS; the CRT at most 35.8 KB), so no attribute
+    // needs raising.
     static_assert(size_t(kEncCoefs) * (256 + 16) + sizeof(EncPrime) * kMaxPrimesCrt <= 65536, "encode LDS");
-    static_assert(size_t(16) * kEfS * sizeof(double) + sizeof(EncPrime) * kMaxPrimesCrt + 64 * (224 + 16) <= 65536,
+    static_assert(size_t(16) * kEfS * sizeof(double) + sizeof(double) * kMaxPrimesCrt + 64 * (224 + 16) <= 65536,
                   "fused encode LDS");
     static_assert(size_t(kCrtCoefs) * (32 * 17 + 16) <= 65536 && size_t(65) * kCrtCoefs * 8 <= 65536, "CRT LDS");
     return hipSuccess;

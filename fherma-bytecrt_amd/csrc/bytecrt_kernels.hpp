@@ -68,6 +68,12 @@ struct GroupDesc {
 struct GroupTable {
     GroupDesc g[kMaxGroups];
     uint32_t n;
+    // Work queue (null: one workgroup per tile).  Otherwise each stage runs
+    // as a resident grid whose workgroups pull tiles from queue[8 s + x],
+    // x = the workgroup's XCD, until total[s] is reached; the counters start
+    // at zero (launch_product_groups clears them with a kernel first).
+    uint32_t* queue;
+    uint32_t total[kGroupStages];
 };
 static_assert(sizeof(GroupTable) <= 4096, "kernel argument limit");
 
@@ -76,8 +82,12 @@ static_assert(sizeof(GroupTable) <= 4096, "kernel argument limit");
 uint32_t group_workgroups(const DevTables& t, uint32_t batch, int stage);
 
 // Enqueue the five stages for groups[0..ngroups): one launch per stage
-// covering every product of every group.
-hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipStream_t stream);
+// covering every product of every group.  With `queue` (kQueueWords device
+// words, owned by the caller, not shared by launches that may overlap) every
+// stage runs as a persistent grid pulling (group, product, prime) tiles.
+constexpr uint32_t kQueueWords = 8 * kGroupStages;
+hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipStream_t stream,
+                                 uint32_t* queue = nullptr);
 
 // Enqueue the whole product pipeline for a batch on `stream` (no host sync).
 hipError_t launch_product(const DevTables& t, const BatchArgs& args, hipStream_t stream);

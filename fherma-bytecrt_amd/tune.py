@@ -200,9 +200,10 @@ def tune_kernel_flags(engine, a, b, c, batch: int, candidates=KERNEL_FLAG_CANDID
 
 # Mix schedules (chunks per geometry class, worker streams, flags): one launch
 # sequence per class on the caller's stream, classes concurrently on two
-# streams, chunked classes over two to eight streams, and MERGE_CLASSES (every
-# class in the same launches, the kernels switching shape per workgroup).
-MIX_CANDIDATES = ((1, 1, 0), (1, 2, 0), (2, 2, 0), (2, 4, 0), (4, 4, 0), (4, 8, 0), (1, 1, 1))
+# streams, chunked classes over two to eight streams, MERGE_CLASSES (1: every
+# class in the same launches, the kernels switching shape per workgroup) and
+# PERSISTENT (2: resident grids pulling tiles from work counters).
+MIX_CANDIDATES = ((1, 1, 0), (1, 2, 0), (2, 2, 0), (2, 4, 0), (4, 4, 0), (4, 8, 0), (1, 1, 1), (1, 1, 2))
 
 
 def tune_mix(mix, outputs, candidates=MIX_CANDIDATES, reps: int = 5, stream=None):

@@ -158,10 +158,16 @@ void bcrt_mix_destroy(bcrt_mix* m);
  * forked from the caller's stream (1: every chunk on the caller's stream, in
  * order).  BCRT_MIX_MERGE_CLASSES puts every class into the same chunks (one
  * launch per stage and chunk over all geometries; the kernels then switch
- * shape per workgroup).  1 <= chunks, streams <= 16; a new mix has chunks =
- * streams = 1, no flags.  Waits for the mix's pending work and drops its
- * graph; results do not depend on the schedule. */
+ * shape per workgroup).  BCRT_MIX_PERSISTENT runs every stage launch as a
+ * resident grid (as many workgroups as the device holds at once) whose
+ * workgroups pull (group, product, prime) tiles from per-XCD work counters
+ * (the mix allocates 192 bytes of counters per chunk; a small kernel clears
+ * them at the start of each chunk, inside the captured graph too).
+ * 1 <= chunks, streams <= 16; a new mix has chunks = streams = 1, no flags.
+ * Waits for the mix's pending work and drops its graph; results do not depend
+ * on the schedule. */
 #define BCRT_MIX_MERGE_CLASSES 1u
+#define BCRT_MIX_PERSISTENT 2u
 int bcrt_mix_set_schedule(bcrt_mix* m, uint32_t chunks, uint32_t streams, uint32_t flags);
 /* Launch sequences (chunks) of the mix's current schedule; BCRT_EINVAL for a
  * null mix. */

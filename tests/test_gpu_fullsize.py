@@ -149,6 +149,16 @@ def test_uneven_batch_config(pkg, orc, gu):
     gu.sync()
     for (*_, out), want in zip(host, first):
         assert torch.equal(out, want), "mix graph replays differ"
+    # the same batch as persistent work-queue launches, classes on two streams
+    mix.set_schedule(1, 2, pkg.Mix.PERSISTENT)
+    for _ in range(2):
+        for *_, out in host:
+            out.zero_()
+        gu.sync()
+        mix.run(graph=True)
+        gu.sync()
+        for (*_, out), want in zip(host, first):
+            assert torch.equal(out, want), "persistent mix differs"
     mix.close()
 
 

@@ -93,7 +93,8 @@ def test_mix_schedules(pkg, orc, gu):
     """26 products of three geometries under every schedule shape: one launch
     sequence per class on the caller's stream (the default), classes dealt
     into chunks on worker streams (each group split across chunks), chunks
-    sharing fewer streams, and every class merged into the same launches;
+    sharing fewer streams, every class merged into the same launches, and
+    persistent work-queue launches (alone, chunked over streams, merged);
     eager and graph replays equal each product computed alone."""
     spec = [(8192, 220, 12), (4096, 100, 9), (16384, 436, 5)]
     groups, host = [], []
@@ -111,8 +112,9 @@ def test_mix_schedules(pkg, orc, gu):
     for e, q, a, b, out in host:
         want.append(np.stack([alone(pkg, gu, e, a[i], b[i]) for i in range(a.shape[0])]))
     # (chunks per class, streams, flags) -> launch sequences: 5 products cap the N=16,384 class
-    for sched, launches in (((1, 1, 0), 3), ((3, 3, 0), 9), ((4, 2, 0), 12), ((2, 1, 0), 6),
-                            ((1, 1, pkg.Mix.MERGE_CLASSES), 1), ((3, 2, pkg.Mix.MERGE_CLASSES), 3)):
+    merge, pq = pkg.Mix.MERGE_CLASSES, pkg.Mix.PERSISTENT
+    for sched, launches in (((1, 1, 0), 3), ((3, 3, 0), 9), ((4, 2, 0), 12), ((2, 1, 0), 6), ((1, 1, merge), 1),
+                            ((3, 2, merge), 3), ((1, 1, pq), 3), ((2, 3, pq), 6), ((2, 2, merge | pq), 2)):
         mix.set_schedule(*sched)
         assert mix.chunks == launches, sched
         for graph in (False, True, True):
@@ -124,7 +126,7 @@ def test_mix_schedules(pkg, orc, gu):
             for (e, *_r, out), w in zip(host, want):
                 np.testing.assert_array_equal(gu.from_dev(out).reshape(-1, e.N, e.L), w,
                                               err_msg=f"N={e.N} schedule={sched} graph={graph}")
-    for bad in ((0, 1, 0), (17, 1, 0), (1, 0, 0), (1, 17, 0), (1, 1, 2)):
+    for bad in ((0, 1, 0), (17, 1, 0), (1, 0, 0), (1, 17, 0), (1, 1, 4)):
         with pytest.raises(ValueError):
             mix.set_schedule(*bad)
     e, q, a, b, _ = host[2]
