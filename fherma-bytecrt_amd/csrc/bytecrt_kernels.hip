@@ -935,6 +935,11 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
 // digits apart across a wave) on distinct banks.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kCrtThreads = 256;
+constexpr int kCrtAhead = 4;  // k-steps the weight loads run ahead of the GEMM
+// Workgroups per CU the CRT kernels are built for: the prefetched weight
+// fragments need up to 128 VGPRs (five per CU left 102 and serialised the
+// loads instead; four with the prefetch were 10% faster at Kcrt = 480).
+constexpr int kCrtWgPerCu = 4;
 constexpr uint32_t kCrtDigitStride = kCrtCoefs + 1;
 
 size_t crt_smem_bytes(const DevTables& t) {
@@ -1027,22 +1032,40 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
     const uint32_t row = lane & 31, kh = lane >> 5;
     const uint32_t MT = t.Mcrt / 32;
     v16i acc0[KT], acc1[KT];
+    const int8_t* wrow[KT];
+    bool live[KT];
 #pragma unroll
     for (int i = 0; i < KT; ++i) {
         acc0[i] = v16i{};
         acc1[i] = v16i{};
         const uint32_t mt = wave + 4 * i;
-        if (mt < MT) {
-            const int8_t* wrow = t.wcrt + size_t(mt * 32 + row) * t.Kcrt + 16 * kh;
-            v4i A[KS];
+        live[i] = mt < MT;
+        wrow[i] = t.wcrt + size_t(min(mt, MT - 1) * 32 + row) * t.Kcrt + 16 * kh;  // clamped: in bounds
+    }
+    // Weight fragments (from L2) are loaded kCrtAhead k-steps ahead of the
+    // MFMAs that use them, for both tiles of the wave at once; left to itself
+    // the compiler issued each load just before its MFMA and waited an L2
+    // round trip per k-step (0.8 ms of a 2.55 ms batch).
+    constexpr int D = kCrtAhead;
+    v4i A[KT][KS];
 #pragma unroll
-            for (int k = 0; k < KS; ++k) A[k] = *reinterpret_cast<const v4i*>(wrow + 32 * k);
+    for (int k = 0; k < KS && k < D; ++k)
 #pragma unroll
-            for (int k = 0; k < KS; ++k) {
-                const v4i B0 = *reinterpret_cast<const v4i*>(tmat + row * tstride + 32 * k + 16 * kh);
-                const v4i B1 = *reinterpret_cast<const v4i*>(tmat + (32 + row) * tstride + 32 * k + 16 * kh);
-                acc0[i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B0, acc0[i], 0, 0, 0);
-                acc1[i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[k], B1, acc1[i], 0, 0, 0);
+        for (int i = 0; i < KT; ++i) A[i][k] = *reinterpret_cast<const v4i*>(wrow[i] + 32 * k);
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+        if (k + D < KS) {
+#pragma unroll
+            for (int i = 0; i < KT; ++i) A[i][k + D] = *reinterpret_cast<const v4i*>(wrow[i] + 32 * (k + D));
+        }
+        __builtin_amdgcn_sched_barrier(0);  // the loads stay ahead of the MFMAs below
+        const v4i B0 = *reinterpret_cast<const v4i*>(tmat + row * tstride + 32 * k + 16 * kh);
+        const v4i B1 = *reinterpret_cast<const v4i*>(tmat + (32 + row) * tstride + 32 * k + 16 * kh);
+#pragma unroll
+        for (int i = 0; i < KT; ++i) {
+            if (live[i]) {
+                acc0[i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][k], B0, acc0[i], 0, 0, 0);
+                acc1[i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][k], B1, acc1[i], 0, 0, 0);
             }
         }
     }
@@ -1283,8 +1306,9 @@ __global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __re
 // allocation (one kernel switching over all seventeen spilled 88 VGPRs; at
 // Kcrt = 480 this one spills none).
 template <int KS>
-__global__ __launch_bounds__(kCrtThreads, 5) void k_crt(DevTables t, const uint32_t* __restrict__ planes,
-                                                     uint32_t* __restrict__ out) {
+__global__ __launch_bounds__(kCrtThreads, kCrtWgPerCu) void k_crt(DevTables t,
+                                                                const uint32_t* __restrict__ planes,
+                                                                uint32_t* __restrict__ out) {
     extern __shared__ __align__(16) uint8_t smem_crt[];
     uint32_t x, y, z;
     single_xyz(t, 4, &x, &y, &z);
@@ -1423,7 +1447,7 @@ __global__ __launch_bounds__(kThreads) void k_inv_cols_groups(const GroupTable g
 }
 
 template <int KS, bool PQ>
-__global__ __launch_bounds__(kCrtThreads, 5) void k_crt_groups(const GroupTable g) {
+__global__ __launch_bounds__(kCrtThreads, kCrtWgPerCu) void k_crt_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_crt_g[];
     for_each_tile<PQ>(g, 4, [](const GroupDesc& d, uint32_t x, uint32_t, uint32_t z) {
         if constexpr (KS == 0)
