@@ -198,69 +198,58 @@ struct EncGeom {
     }
 };
 
-constexpr uint32_t kEncSteps = kEncCoefs * 64 / kThreads;  // input words per thread (L <= 64)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// Word v of the tile (v = threadIdx.x + kThreads * it, in load order) is limb
-// v % L of the tile's coefficient c = v / L: group gi = c >> (7 - ts_log) (one
-// s), row ro = c & (TR - 1) within it.  The groups' TR * L words are contiguous
-// and N2 * L apart, so word v sits at source offset v + gi (N2 - TR) L and in
-// LDS row (ro << ts_log) + gi.  (c, limb) advance by (kThreads / L,
-// kThreads % L) per step with at most one carry, so no word needs a division.
-struct EncWalk {
-    uint32_t c, limb, dq, dr;
-    __device__ __forceinline__ explicit EncWalk(uint32_t L) {
-        c = threadIdx.x / L;
-        limb = threadIdx.x - c * L;
-        dq = kThreads / L;
-        dr = kThreads - dq * L;
-    }
-    __device__ __forceinline__ void step(uint32_t L) {
-        c += dq;
-        limb += dr;
-        const bool wrap = limb >= L;
-        limb = wrap ? limb - L : limb;
-        c += wrap ? 1u : 0u;
+// Input staging of an encode tile.  In LDS each of the tile's 128
+// coefficients has a row of Kenc bytes (8 KS words: its L limbs, then
+// padding), moved as 2 KS pieces of 16 bytes, KS per thread: one buffer load
+// and one 16-byte LDS store per piece, every load in flight before the first
+// store.  Piece P = threadIdx.x + kThreads * it is piece q = P mod 2 KS of tile
+// coefficient c = P / 2 KS, which is row ro = c & (TR - 1) of group gi =
+// c >> (7 - ts_log) (one s; a group's TR coefficients are contiguous in the
+// operand, groups N2 L words apart): it starts at source word
+// (ro + N2 gi) L + 4 q from the tile's first coefficient and lands at word 4 q
+// of LDS row (ro << ts_log) + gi.  The padding words of a row receive
+// whatever follows the coefficient in the operand (the next coefficient's
+// first limbs, or 0 past the operand's end, where buffer loads return 0):
+// the plan fills only columns k < 4 L of the weight tiles, so padding bytes
+// add nothing to any dot sum.  The loads are raw buffer loads with 32-bit
+// offsets (N L < 2^22) over the operand from the tile's first coefficient on.
+template <int KS>
+struct EncPiece {
+    uint32_t src, row;  // source word offset, LDS row
+    __device__ __forceinline__ EncPiece(const DevTables& t, const EncGeom& e, uint32_t it) {
+        constexpr uint32_t PPC = 2 * KS;
+        // P / PPC for P < 2^12 as a 24-bit multiply by ceil(2^20 / PPC) (exact:
+        // the rounding adds less than 2^-8 < 1 / PPC)
+        constexpr uint32_t kInv = ((1u << 20) + PPC - 1) / PPC;
+        const uint32_t P = threadIdx.x + kThreads * it;
+        const uint32_t c = __umul24(P, kInv) >> 20, q = P - c * PPC;
+        const uint32_t tr_log = 7 - e.ts_log, gi = c >> tr_log, ro = c & (e.TR - 1);
+        src = __umul24(ro + __umul24(gi, t.N2), t.L) + 4 * q;
+        row = __umul24((ro << e.ts_log) + gi, e.stride_words) + 4 * q;
     }
 };
 
-// Issues every input load of the tile into registers.  Loads are
-// unconditional (past the end a thread re-reads the tile's last word, whose
-// coefficient 127 is in the last group) so all are in flight at once.
-// Offsets stay 32-bit: N * L < 2^22; gi < 128 and (N2 - TR) L < 2^14.
+template <int KS>
 __device__ __forceinline__ void enc_load(const DevTables& t, const uint32_t* __restrict__ in, const EncGeom& e,
-                                         uint32_t (&regs)[kEncSteps]) {
-    const uint32_t L = t.L, total = kEncCoefs * L, tr_log = 7 - e.ts_log;
-    const uint32_t D = (t.N2 - e.TR) * L;
-    const uint32_t* src = in + (size_t(e.r0) + size_t(t.N2) * e.s0) * L;
-    EncWalk w(L);
+                                         u32x4 (&regs)[KS]) {
+    const uint32_t first = (e.r0 + t.N2 * e.s0) * t.L;  // word offset of the tile in the operand
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(in + first), 0, int((t.N * t.L - first) * 4), 0x00020000);
 #pragma unroll
-    for (uint32_t it = 0; it < kEncSteps; ++it) {
-        const uint32_t v = min(threadIdx.x + kThreads * it, total - 1);
-        const uint32_t gi = min(w.c, kEncCoefs - 1) >> tr_log;
-        regs[it] = src[v + __umul24(gi, D)];
-        w.step(L);
-    }
+    for (uint32_t it = 0; it < KS; ++it)
+        regs[it] = __builtin_amdgcn_raw_buffer_load_b128(rs, 4 * EncPiece<KS>(t, e, it).src, 0, 0);
 }
 
-// Writes the loaded words into the LDS tile as biased bytes (x ^ 0x80 =
-// x - 128, for the signed MFMA); padding limbs L .. Kenc/4 - 1 hold biased
-// zeros.  Words past the tile go to row 0's first padding word (column
-// Kenc/4), which the GEMM never reads, so every store is unconditional.
-__device__ __forceinline__ void enc_store(const DevTables& t, const EncGeom& e, const uint32_t (&regs)[kEncSteps],
+// The loaded pieces into the LDS tile as biased bytes (x ^ 0x80 = x - 128, for
+// the signed MFMA).
+template <int KS>
+__device__ __forceinline__ void enc_store(const DevTables& t, const EncGeom& e, const u32x4 (&regs)[KS],
                                           uint32_t* enc_tile) {
-    const uint32_t L = t.L, total = kEncCoefs * L, tr_log = 7 - e.ts_log, tr_mask = e.TR - 1;
-    EncWalk w(L);
 #pragma unroll
-    for (uint32_t it = 0; it < kEncSteps; ++it) {
-        const uint32_t v = threadIdx.x + kThreads * it;
-        const uint32_t m = ((w.c & tr_mask) << e.ts_log) + (w.c >> tr_log);
-        const uint32_t idx = v < total ? __umul24(m, e.stride_words) + w.limb : e.row_words;
-        enc_tile[idx] = regs[it] ^ 0x80808080u;
-        w.step(L);
-    }
-    const uint32_t pad = e.row_words - L, row = threadIdx.x & (kEncCoefs - 1);
-    for (uint32_t k = threadIdx.x / kEncCoefs; k < pad; k += kThreads / kEncCoefs)
-        enc_tile[__umul24(row, e.stride_words) + L + k] = 0x80808080u;
+    for (uint32_t it = 0; it < KS; ++it)
+        *reinterpret_cast<u32x4*>(enc_tile + EncPiece<KS>(t, e, it).row) = regs[it] ^ 0x80808080u;
 }
 
 // The per-prime reduction constants {off, 1/p rounded down, p, off mod 2^32}
@@ -358,9 +347,9 @@ __device__ __forceinline__ void encode_wg(const DevTables& t, const uint32_t* __
                                           uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem) {
     const EncPrime* cst = enc_constants(t, smem);
     const EncGeom e(t, bx);
-    uint32_t regs[kEncSteps];
-    enc_load(t, (op ? b : a) + size_t(prod) * t.N * t.L, e, regs);
-    enc_store(t, e, regs, reinterpret_cast<uint32_t*>(smem));
+    u32x4 regs[KS];
+    enc_load<KS>(t, (op ? b : a) + size_t(prod) * t.N * t.L, e, regs);
+    enc_store<KS>(t, e, regs, reinterpret_cast<uint32_t*>(smem));
     __syncthreads();
     enc_compute<KS>(t, planes, e, op, prod, smem, cst);
 }
@@ -397,6 +386,22 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also
+// makes every wave wait until its global stores are acknowledged
+// (s_waitcnt vmcnt(0) before s_barrier); this one waits for LDS operations
+// (lgkmcnt) alone, so stores issued just before it drain in the background.
+// Direct global -> LDS loads are vector-memory operations: a wave that issued
+// some waits for them itself (wait_vmem) before such a barrier.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// s_waitcnt vmcnt(0): every vector-memory operation of this wave (loads,
+// direct global -> LDS loads, stores) has completed.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 template <uint32_t M>
 constexpr uint32_t exchange_stride() {
@@ -749,12 +754,15 @@ __device__ __forceinline__ void encfwd_body(const DevTables& t, const uint32_t* 
     auto load_weights = [&](uint32_t grp) {
         if constexpr (!stage_w) return;
         constexpr uint32_t SPR = 2 * KS + 1;
+        // k / SPR for k < 2^12 as a 24-bit multiply by ceil(2^20 / SPR): exact,
+        // since k (2^20 / SPR rounded up - 2^20 / SPR) / 2^20 < 2^-8 < 1 / SPR
+        constexpr uint32_t kInvSpr = ((1u << 20) + SPR - 1) / SPR;
         const uint32_t slots = min(64u, (t.PTenc - 2 * grp) * 32) * SPR;
         const int8_t* src = t.wenc + size_t(2 * grp) * 32 * (32 * KS);
         for (uint32_t n = wave; n * 64 < slots; n += 4) {
             const uint32_t k = n * 64 + lane;
             if (k < slots) {
-                const uint32_t rw = k / SPR, c16 = min(k - rw * SPR, 2 * KS - 1);
+                const uint32_t rw = __umul24(k, kInvSpr) >> 20, c16 = min(k - __umul24(rw, SPR), 2 * KS - 1);
                 __builtin_amdgcn_global_load_lds(src + rw * (32 * KS) + 16 * c16, (lds_void_ptr)(wbuf + n * 1024),
                                                  16, 0, 0);
             }
@@ -784,10 +792,10 @@ __device__ __forceinline__ void encfwd_body(const DevTables& t, const uint32_t* 
     for (uint32_t h = 0; h < 2; ++h) {
         // half h: coefficients 128 h .. 128 h + 127 of the workgroup
         const EncGeom e = NLOG1 == 8 ? EncGeom(t, 7, r0, kEncCoefs * h) : EncGeom(t, ts_log, r0 + h * TRh, 0);
-        uint32_t regs[kEncSteps];
-        enc_load(t, in, e, regs);
+        u32x4 regs[KS];
+        enc_load<KS>(t, in, e, regs);
         if (h) __syncthreads();  // the first half's fragments have been read
-        enc_store(t, e, regs, reinterpret_cast<uint32_t*>(smem));
+        enc_store<KS>(t, e, regs, reinterpret_cast<uint32_t*>(smem));
         __syncthreads();
         if ((wave >> 1) == h) {
             const uint32_t c0 = 2 * (wave & 1);
@@ -871,6 +879,12 @@ __device__ __forceinline__ void encfwd_body(const DevTables& t, const uint32_t* 
             for (int x = 0; x < 16; ++x) xc[skew(tau + T * x)] = v[x];
         }
         wave_lds_sync();
+        // This wave's direct-to-LDS loads of the next pair's weight tiles
+        // (issued before the first round) are complete from here on (the
+        // second round waits for its twiddle loads anyway), so the barrier at
+        // the end of the pair waits for LDS operations only, not for the
+        // stores below.
+        wait_vmem();
         if (live) {
 #pragma unroll
             for (int x = 0; x < 16; ++x) v[x] = xc[skew(16 * tau + x)];
@@ -885,7 +899,7 @@ __device__ __forceinline__ void encfwd_body(const DevTables& t, const uint32_t* 
                     make_uint4(uint32_t(balanced(v[x], p, pinv)), uint32_t(balanced(v[x + 1], p, pinv)),
                                uint32_t(balanced(v[x + 2], p, pinv)), uint32_t(balanced(v[x + 3], p, pinv)));
         }
-        __syncthreads();  // the exchange, constants and weights are rewritten for the next pair
+        lds_barrier();  // the exchange, constants and weights are rewritten for the next pair
     }
 }
 
