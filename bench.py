@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--sweep", action="store_true",
                     help="autotune the batch pipelining, write the pipeline table and exit")
+    ap.add_argument("--sweep-geometries", default="65536:1747,32768:868,16384:436,8192:220",
+                    help="N:W pairs --sweep tunes (B products each)")
     ap.add_argument("--sweep-out", default="pipeline_table.json",
                     help="where --sweep writes its table (copy it over fherma-bytecrt_amd/pipeline_table.json to use it)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -433,42 +435,55 @@ def cpu_baseline(pkg, q, seconds: float):
 
 def sweep_mode(args, torch):
     """Autotune the encode tile shape, the fusion of encode with the forward
-    columns, the kernel flags and then the batch pipelining for the bench geometry (every
-    candidate's product is checked bit for bit against the first) and record
-    the choices in a pipeline table (copy it into the package to make it the
-    default)."""
+    columns, the kernel flags and then the batch pipelining for B products at
+    each geometry of --sweep-geometries (the bench geometry and the other
+    configs' N, W; every candidate's product is checked bit for bit against
+    the first) and record the choices in a pipeline table (copy it into the
+    package to make it the default)."""
     sys.path.insert(0, str(ROOT))
     import __graft_entry__ as ge
     pkg = ge.load_package()
     import bytecrt_amd.tune as tune
-    q = pkg.datagen.modulus(W, 2026)
-    eng = pkg.Engine(N, W, q, max_batch=B)
-    a = device_operands(torch, B, N, q, 11)
-    b = device_operands(torch, B, N, q, 12)
+    out = Path(args.sweep_out)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    for geo in args.sweep_geometries.split(","):
+        n, w = (int(x) for x in geo.split(":"))
+        sweep_geometry(args, torch, pkg, tune, n, w, B, out)
+
+
+def sweep_geometry(args, torch, pkg, tune, n: int, w: int, batch: int, out: Path):
+    """Tune the encode tile, the fusion and the kernel flags (and the
+    pipelining, while its tuner is on) for ``batch`` products at (n, w) and
+    record them in the table at ``out``."""
+    q = pkg.datagen.modulus(w, 2026)
+    eng = pkg.Engine(n, w, q, max_batch=batch)
+    a = device_operands(torch, batch, n, q, 11)
+    b = device_operands(torch, batch, n, q, 12)
     c = torch.empty_like(a)
     torch.cuda.synchronize()
     stream = torch.cuda.Stream()
-    enc = tune.tune_encode_tile(eng, a, b, c, B, reps=args.reps, stream=stream)
+    tag = {"N": n, "W": w, "B": batch}
+    enc = tune.tune_encode_tile(eng, a, b, c, batch, reps=args.reps, stream=stream)
     for r in enc["rows"]:
-        print(json.dumps(r), flush=True)
-    fus = tune.tune_fusion(eng, a, b, c, B, reps=args.reps, stream=stream)
+        print(json.dumps({**tag, **r}), flush=True)
+    fus = tune.tune_fusion(eng, a, b, c, batch, reps=args.reps, stream=stream)
     for r in fus["rows"]:
-        print(json.dumps(r), flush=True)
-    kfl = tune.tune_kernel_flags(eng, a, b, c, B, reps=args.reps, stream=stream)
+        print(json.dumps({**tag, **r}), flush=True)
+    kfl = tune.tune_kernel_flags(eng, a, b, c, batch, reps=args.reps, stream=stream)
     for r in kfl["rows"]:
-        print(json.dumps(r), flush=True)
+        print(json.dumps({**tag, **r}), flush=True)
     try:
-        res = tune.tune_pipeline(eng, a, b, c, B, reps=args.reps, stream=stream)
+        res = tune.tune_pipeline(eng, a, b, c, batch, reps=args.reps, stream=stream)
     except RuntimeError as x:  # the pipeline tuner is switched off
         print(json.dumps({"pipeline": str(x)}), flush=True)
         res = None
     for r in (res or {}).get("rows", []):
-        print(json.dumps(r), flush=True)
-    out = Path(args.sweep_out)
-    out.parent.mkdir(parents=True, exist_ok=True)
-    tune.record(N, W, B, res, device=torch.cuda.get_device_name(0), path=out, encode=enc, fusion=fus, flags=kfl)
-    print(json.dumps({"best": res and res["best"], "encode_tile": enc["best"], "fusion": fus["best"],
+        print(json.dumps({**tag, **r}), flush=True)
+    tune.record(n, w, batch, res, device=torch.cuda.get_device_name(0), path=out, encode=enc, fusion=fus, flags=kfl)
+    print(json.dumps({**tag, "best": res and res["best"], "encode_tile": enc["best"], "fusion": fus["best"],
                       "kernel_flags": kfl["best"], "table": str(out)}), flush=True)
+    del eng, a, b, c
+    torch.cuda.empty_cache()
 
 
 def main():

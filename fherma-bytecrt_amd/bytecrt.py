@@ -248,12 +248,15 @@ class Engine:
         self._check(load_library().bcrt_set_fusion(self._h, min_tiles))
         self.fuse_min_tiles = min_tiles
 
-    FLAG_NO_XCD, FLAG_NO_PRIO, FLAG_WEIGHTS_L2 = 1, 2, 4
+    FLAG_NO_XCD, FLAG_NO_PRIO, FLAG_WEIGHTS_L2, FLAG_CRT_WIDE, FLAG_ENC_ONE_CHAIN = 1, 2, 4, 8, 16
 
     def set_kernel_flags(self, flags: int):
         """Kernel-level choices (bytecrt.h bcrt_set_kernel_flags): FLAG_NO_XCD
         plain workgroup order, FLAG_NO_PRIO no raised MFMA issue priority in the
-        fused encode, FLAG_WEIGHTS_L2 fused-encode weights from L2."""
+        fused encode, FLAG_WEIGHTS_L2 fused-encode weights from L2,
+        FLAG_CRT_WIDE the CRT built for five workgroups per CU (weights one
+        k-step ahead), FLAG_ENC_ONE_CHAIN the separate encode with one MFMA
+        chain per wave at four workgroups per CU."""
         self._check(load_library().bcrt_set_kernel_flags(self._h, flags))
         self.kernel_flags = flags
 

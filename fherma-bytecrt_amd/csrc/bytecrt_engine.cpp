@@ -522,7 +522,7 @@ extern "C" int bcrt_set_encode_tile(bcrt_engine* e, uint32_t ts_log) {
 extern "C" int bcrt_set_kernel_flags(bcrt_engine* e, uint32_t flags) {
     if (!e) return BCRT_EINVAL;
     return guarded(e, [&] {
-        if (flags > bcrt::kFlagsAll) throw std::invalid_argument("ByteCRT kernel flags must be within 0..7");
+        if (flags > bcrt::kFlagsAll) throw std::invalid_argument("ByteCRT kernel flags must be within 0..31");
         drop_graph(e);  // captured graphs hold the tables by value
         e->tables.kflags = flags;
     });

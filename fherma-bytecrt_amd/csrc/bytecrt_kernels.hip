@@ -40,7 +40,6 @@ constexpr uint32_t kEncCoefs = 128;  // coefficients per encode workgroup
 constexpr uint32_t kCrtCoefs = 64;   // coefficients per CRT workgroup
 constexpr uint32_t kMidBlocks = 16;  // 256-element blocks per middle workgroup
 constexpr uint32_t kMaxPrimesCrt = 128;
-constexpr int kEncChains = 2;        // independent MFMA accumulator chains in encode
 
 // ---------------------------------------------------------------------------
 // Exact modular arithmetic in FP64, p in (2^30, 2^31).
@@ -261,8 +260,13 @@ __device__ __forceinline__ EncPrime* enc_constants(const DevTables& t, uint8_t* 
     return cst;
 }
 
-// GEMM and reduction of the tile in LDS, residues to the planes.
-template <int KS>
+// GEMM and reduction of the tile in LDS, residues to the planes.  C = 2: wave w
+// takes the coefficient sub-tiles 2 (w & 1) and 2 (w & 1) + 1 (two independent
+// accumulator chains) and the weight tiles pt = w >> 1, +2, ...; C = 1: wave w
+// takes sub-tile w and every weight tile (half the accumulators, so the
+// kernel fits four workgroups per CU, but each weight tile is loaded by four
+// waves instead of two).
+template <int KS, int C>
 __device__ __forceinline__ void enc_compute(const DevTables& t, uint32_t* __restrict__ planes, const EncGeom& e,
                                             uint32_t op, uint32_t prod, const uint8_t* smem, const EncPrime* cst) {
     const uint32_t* enc_tile = reinterpret_cast<const uint32_t*>(smem);
@@ -272,22 +276,20 @@ __device__ __forceinline__ void enc_compute(const DevTables& t, uint32_t* __rest
     const uint8_t* tile_bytes = reinterpret_cast<const uint8_t*>(enc_tile);
     const uint32_t stride_bytes = stride_words * 4;
     uint32_t* out = plane_ptr(t, planes, prod, op, 0);  // prime j at out + j * N (P * N < 2^32)
-    // Wave w takes the coefficient sub-tiles 2 (w & 1) and 2 (w & 1) + 1 (their
-    // data fragments stay in registers) and weight tiles pt = w >> 1, +2, ...;
-    // the two sub-tiles accumulate in independent chains.
-    const uint32_t c0 = 2 * (wave & 1);
+    static_assert(C == 1 || C == 2, "encode chains");
+    const uint32_t c0 = C == 2 ? 2 * (wave & 1) : wave, pt0 = C == 2 ? wave >> 1 : 0, pstep = C;
     // weight fragments of the next tile are loaded while this one computes
     v4i A[KS];
     {
-        const uint32_t pt = min(wave >> 1, t.PTenc - 1);
+        const uint32_t pt = min(pt0, t.PTenc - 1);
         const int8_t* wrow = t.wenc + (size_t(pt) * 32 + row) * t.Kenc + 16 * kh;
 #pragma unroll
         for (int s2 = 0; s2 < KS; ++s2) A[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
     }
-    for (uint32_t pt = wave >> 1; pt < t.PTenc; pt += 2) {
+    for (uint32_t pt = pt0; pt < t.PTenc; pt += pstep) {
         v4i An[KS];
         {
-            const uint32_t pn = min(pt + 2, t.PTenc - 1);  // clamped: the last tile reloads itself
+            const uint32_t pn = min(pt + pstep, t.PTenc - 1);  // clamped: the last tile reloads itself
             const int8_t* wrow = t.wenc + (size_t(pn) * 32 + row) * t.Kenc + 16 * kh;
 #pragma unroll
             for (int s2 = 0; s2 < KS; ++s2) An[s2] = *reinterpret_cast<const v4i*>(wrow + 32 * s2);
@@ -302,22 +304,22 @@ __device__ __forceinline__ void enc_compute(const DevTables& t, uint32_t* __rest
             inv[g] = c.inv;
             pd[g] = double(c.p);
         }
-        v16i acc[kEncChains];
+        v16i acc[C];
 #pragma unroll
-        for (int c = 0; c < kEncChains; ++c) acc[c] = v16i{};
+        for (int c = 0; c < C; ++c) acc[c] = v16i{};
 #pragma unroll
         for (int s2 = 0; s2 < KS; ++s2) {
-            v4i B[kEncChains];
+            v4i B[C];
 #pragma unroll
-            for (int c = 0; c < kEncChains; ++c)
+            for (int c = 0; c < C; ++c)
                 B[c] = *reinterpret_cast<const v4i*>(tile_bytes + ((c0 + c) * 32 + row) * stride_bytes + 16 * kh +
                                                      32 * s2);
 #pragma unroll
-            for (int c = 0; c < kEncChains; ++c)
+            for (int c = 0; c < C; ++c)
                 acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[s2], B[c], acc[c], 0, 0, 0);
         }
 #pragma unroll
-        for (int c = 0; c < kEncChains; ++c) {
+        for (int c = 0; c < C; ++c) {
             const uint32_t m = (c0 + c) * 32 + row;
             const uint32_t f = (r0 + (m >> ts_log)) * t.N1 + s0 + (m & (TS - 1));
 #pragma unroll
@@ -340,8 +342,8 @@ __device__ __forceinline__ void enc_compute(const DevTables& t, uint32_t* __rest
     }
 }
 
-// One encode tile (bx, op, prod).
-template <int KS>
+// One encode tile (bx, op, prod) with C MFMA chains per wave.
+template <int KS, int C = 2>
 __device__ __forceinline__ void encode_wg(const DevTables& t, const uint32_t* __restrict__ a,
                                           const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
                                           uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem) {
@@ -351,22 +353,23 @@ __device__ __forceinline__ void encode_wg(const DevTables& t, const uint32_t* __
     enc_load<KS>(t, (op ? b : a) + size_t(prod) * t.N * t.L, e, regs);
     enc_store<KS>(t, e, regs, reinterpret_cast<uint32_t*>(smem));
     __syncthreads();
-    enc_compute<KS>(t, planes, e, op, prod, smem, cst);
+    enc_compute<KS, C>(t, planes, e, op, prod, smem, cst);
 }
 
 // Encode with the k-step count (Kenc / 32, 1..8) as a compile-time constant.
+template <int C = 2>
 __device__ __forceinline__ void encode_dispatch(const DevTables& t, const uint32_t* __restrict__ a,
                                                 const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
                                                 uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem) {
     switch (t.Kenc / 32) {
-        case 1: encode_wg<1>(t, a, b, planes, bx, op, prod, smem); break;
-        case 2: encode_wg<2>(t, a, b, planes, bx, op, prod, smem); break;
-        case 3: encode_wg<3>(t, a, b, planes, bx, op, prod, smem); break;
-        case 4: encode_wg<4>(t, a, b, planes, bx, op, prod, smem); break;
-        case 5: encode_wg<5>(t, a, b, planes, bx, op, prod, smem); break;
-        case 6: encode_wg<6>(t, a, b, planes, bx, op, prod, smem); break;
-        case 7: encode_wg<7>(t, a, b, planes, bx, op, prod, smem); break;
-        default: encode_wg<8>(t, a, b, planes, bx, op, prod, smem); break;
+        case 1: encode_wg<1, C>(t, a, b, planes, bx, op, prod, smem); break;
+        case 2: encode_wg<2, C>(t, a, b, planes, bx, op, prod, smem); break;
+        case 3: encode_wg<3, C>(t, a, b, planes, bx, op, prod, smem); break;
+        case 4: encode_wg<4, C>(t, a, b, planes, bx, op, prod, smem); break;
+        case 5: encode_wg<5, C>(t, a, b, planes, bx, op, prod, smem); break;
+        case 6: encode_wg<6, C>(t, a, b, planes, bx, op, prod, smem); break;
+        case 7: encode_wg<7, C>(t, a, b, planes, bx, op, prod, smem); break;
+        default: encode_wg<8, C>(t, a, b, planes, bx, op, prod, smem); break;
     }
 }
 
@@ -954,6 +957,10 @@ constexpr int kCrtAhead = 4;  // k-steps the weight loads run ahead of the GEMM
 // fragments need up to 128 VGPRs (five per CU left 102 and serialised the
 // loads instead; four with the prefetch were 10% faster at Kcrt = 480).
 constexpr int kCrtWgPerCu = 4;
+// The alternative the autotuner may pick (kFlagCrtWide): five workgroups per
+// CU, with the weight fragments loaded one k-step ahead.
+constexpr int kCrtAheadWide = 1;
+constexpr int kCrtWgPerCuWide = 5;
 constexpr uint32_t kCrtDigitStride = kCrtCoefs + 1;
 
 size_t crt_smem_bytes(const DevTables& t) {
@@ -975,9 +982,10 @@ __device__ __forceinline__ double limb_sum(const v16i& a, int g) {
                fma(double(a[4 * g + 2]), 65536.0, fma(double(a[4 * g + 1]), 256.0, double(a[4 * g]))));
 }
 
-// CRT for 64 coefficients with the k-loop of the GEMM (Kcrt / 32 steps) and the
-// row tiles per wave (KT, four waves) fixed.
-template <int KS, int KT>
+// CRT for 64 coefficients with the k-loop of the GEMM (Kcrt / 32 steps), the
+// row tiles per wave (KT, four waves) and the weight look-ahead (D k-steps)
+// fixed.
+template <int KS, int KT, int D>
 __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __restrict__ planes,
                                        uint32_t* __restrict__ out, uint32_t bx, uint32_t prod,
                                        uint8_t* crt_smem) {
@@ -1056,11 +1064,10 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
         live[i] = mt < MT;
         wrow[i] = t.wcrt + size_t(min(mt, MT - 1) * 32 + row) * t.Kcrt + 16 * kh;  // clamped: in bounds
     }
-    // Weight fragments (from L2) are loaded kCrtAhead k-steps ahead of the
-    // MFMAs that use them, for both tiles of the wave at once; left to itself
-    // the compiler issued each load just before its MFMA and waited an L2
-    // round trip per k-step (0.8 ms of a 2.55 ms batch).
-    constexpr int D = kCrtAhead;
+    // Weight fragments (from L2) are loaded D k-steps ahead of the MFMAs that
+    // use them, for both tiles of the wave at once; left to itself the
+    // compiler issued each load just before its MFMA and waited an L2 round
+    // trip per k-step (0.8 ms of a 2.55 ms batch).
     v4i A[KT][KS];
 #pragma unroll
     for (int k = 0; k < KS && k < D; ++k)
@@ -1166,20 +1173,20 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
     }
 }
 
-// CRT with the k-step count (Kcrt / 32, 1..17) and the row tiles per wave as
-// compile-time constants.
-template <int KS>
+// CRT with the k-step count (Kcrt / 32, 1..17), the row tiles per wave and
+// the weight look-ahead as compile-time constants.
+template <int KS, int D = kCrtAhead>
 __device__ __forceinline__ void crt_tiles(const DevTables& t, const uint32_t* __restrict__ planes,
                                           uint32_t* __restrict__ out, uint32_t bx, uint32_t prod, uint8_t* smem) {
     if constexpr (KS <= 8) {
-        crt_wg<KS, 1>(t, planes, out, bx, prod, smem);
+        crt_wg<KS, 1, D>(t, planes, out, bx, prod, smem);
     } else if constexpr (KS == 9) {
         if (t.Mcrt / 32 <= 4)
-            crt_wg<KS, 1>(t, planes, out, bx, prod, smem);
+            crt_wg<KS, 1, D>(t, planes, out, bx, prod, smem);
         else
-            crt_wg<KS, 2>(t, planes, out, bx, prod, smem);
+            crt_wg<KS, 2, D>(t, planes, out, bx, prod, smem);
     } else {
-        crt_wg<KS, 2>(t, planes, out, bx, prod, smem);
+        crt_wg<KS, 2, D>(t, planes, out, bx, prod, smem);
     }
 }
 
@@ -1270,13 +1277,15 @@ __device__ __forceinline__ void single_xyz(const DevTables& t, int stage, uint32
 // groups whose tiles are numbered group after group, so one launch per stage
 // covers products of different geometries and moduli.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads, 3) void k_encode(DevTables t, const uint32_t* __restrict__ a,
-                                                     const uint32_t* __restrict__ b,
-                                                     uint32_t* __restrict__ planes) {
+// C MFMA chains per wave (kFlagEncOneChain: C = 1, four workgroups per CU).
+template <int C>
+__global__ __launch_bounds__(kThreads, C == 1 ? 4 : 3) void k_encode(DevTables t, const uint32_t* __restrict__ a,
+                                                                  const uint32_t* __restrict__ b,
+                                                                  uint32_t* __restrict__ planes) {
     extern __shared__ __align__(16) uint8_t smem_enc[];
     uint32_t x, y, z;
     single_xyz(t, 0, &x, &y, &z);
-    encode_dispatch(t, a, b, planes, x, y, z, smem_enc);
+    encode_dispatch<C>(t, a, b, planes, x, y, z, smem_enc);
 }
 
 template <int KS, int NLOG1>
@@ -1318,15 +1327,14 @@ __global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __re
 
 // One kernel per k-step count, chosen on the host: each gets its own register
 // allocation (one kernel switching over all seventeen spilled 88 VGPRs; at
-// Kcrt = 480 this one spills none).
-template <int KS>
-__global__ __launch_bounds__(kCrtThreads, kCrtWgPerCu) void k_crt(DevTables t,
-                                                                const uint32_t* __restrict__ planes,
-                                                                uint32_t* __restrict__ out) {
+// Kcrt = 480 this one spills none).  WIDE: the five-per-CU build.
+template <int KS, bool WIDE>
+__global__ __launch_bounds__(kCrtThreads, WIDE ? kCrtWgPerCuWide : kCrtWgPerCu) void k_crt(
+    DevTables t, const uint32_t* __restrict__ planes, uint32_t* __restrict__ out) {
     extern __shared__ __align__(16) uint8_t smem_crt[];
     uint32_t x, y, z;
     single_xyz(t, 4, &x, &y, &z);
-    crt_tiles<KS>(t, planes, out, x, z, smem_crt);
+    crt_tiles<KS, WIDE ? kCrtAheadWide : kCrtAhead>(t, planes, out, x, z, smem_crt);
 }
 
 // Finds the group of workgroup slot `id` of a mixed launch and the slot's
@@ -1507,8 +1515,12 @@ hipError_t dispatch_cols(const DevTables& t, const BatchArgs& args, hipStream_t 
 hipError_t launch_encode(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     if (!shapes_ok(t, args)) return hipErrorInvalidValue;
     const dim3 grid(t.N / kEncCoefs * 2 * args.batch);
-    hipLaunchKernelGGL(k_encode, grid, dim3(kThreads), enc_smem_bytes(t), stream, t, args.a, args.b,
-                       args.planes);
+    if (t.kflags & kFlagEncOneChain)
+        hipLaunchKernelGGL(k_encode<1>, grid, dim3(kThreads), enc_smem_bytes(t), stream, t, args.a, args.b,
+                           args.planes);
+    else
+        hipLaunchKernelGGL(k_encode<2>, grid, dim3(kThreads), enc_smem_bytes(t), stream, t, args.a, args.b,
+                           args.planes);
     return hipGetLastError();
 }
 
@@ -1562,29 +1574,37 @@ hipError_t launch_inverse_columns(const DevTables& t, const BatchArgs& args, hip
     return dispatch_cols(t, args, stream, false);
 }
 
-hipError_t launch_crt(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
-    if (!shapes_ok(t, args)) return hipErrorInvalidValue;
+template <bool WIDE>
+void launch_crt_ks(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     const dim3 g(t.N / kCrtCoefs * args.batch), blk(kCrtThreads);
     const size_t sm = crt_smem_bytes(t);
     switch (t.Kcrt / 32) {
-        case 1: hipLaunchKernelGGL(k_crt<1>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 2: hipLaunchKernelGGL(k_crt<2>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 3: hipLaunchKernelGGL(k_crt<3>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 4: hipLaunchKernelGGL(k_crt<4>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 5: hipLaunchKernelGGL(k_crt<5>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 6: hipLaunchKernelGGL(k_crt<6>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 7: hipLaunchKernelGGL(k_crt<7>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 8: hipLaunchKernelGGL(k_crt<8>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 9: hipLaunchKernelGGL(k_crt<9>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 10: hipLaunchKernelGGL(k_crt<10>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 11: hipLaunchKernelGGL(k_crt<11>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 12: hipLaunchKernelGGL(k_crt<12>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 13: hipLaunchKernelGGL(k_crt<13>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 14: hipLaunchKernelGGL(k_crt<14>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 15: hipLaunchKernelGGL(k_crt<15>, g, blk, sm, stream, t, args.planes, args.out); break;
-        case 16: hipLaunchKernelGGL(k_crt<16>, g, blk, sm, stream, t, args.planes, args.out); break;
-        default: hipLaunchKernelGGL(k_crt<17>, g, blk, sm, stream, t, args.planes, args.out); break;
+        case 1: hipLaunchKernelGGL((k_crt<1, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 2: hipLaunchKernelGGL((k_crt<2, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 3: hipLaunchKernelGGL((k_crt<3, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 4: hipLaunchKernelGGL((k_crt<4, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 5: hipLaunchKernelGGL((k_crt<5, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 6: hipLaunchKernelGGL((k_crt<6, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 7: hipLaunchKernelGGL((k_crt<7, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 8: hipLaunchKernelGGL((k_crt<8, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 9: hipLaunchKernelGGL((k_crt<9, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 10: hipLaunchKernelGGL((k_crt<10, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 11: hipLaunchKernelGGL((k_crt<11, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 12: hipLaunchKernelGGL((k_crt<12, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 13: hipLaunchKernelGGL((k_crt<13, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 14: hipLaunchKernelGGL((k_crt<14, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 15: hipLaunchKernelGGL((k_crt<15, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        case 16: hipLaunchKernelGGL((k_crt<16, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
+        default: hipLaunchKernelGGL((k_crt<17, WIDE>), g, blk, sm, stream, t, args.planes, args.out); break;
     }
+}
+
+hipError_t launch_crt(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
+    if (!shapes_ok(t, args)) return hipErrorInvalidValue;
+    if (t.kflags & kFlagCrtWide)
+        launch_crt_ks<true>(t, args, stream);
+    else
+        launch_crt_ks<false>(t, args, stream);
     return hipGetLastError();
 }
 

@@ -35,7 +35,9 @@ struct DevTables {
 constexpr uint32_t kFlagNoXcd = 1;      // plain workgroup order instead of the XCD tile renumbering
 constexpr uint32_t kFlagNoPrio = 2;     // fused encode: no raised issue priority over its MFMA block
 constexpr uint32_t kFlagWeightsL2 = 4;  // fused encode: weight fragments from L2, not staged in LDS
-constexpr uint32_t kFlagsAll = 7;
+constexpr uint32_t kFlagCrtWide = 8;    // CRT built for five workgroups per CU, weights one k-step ahead
+constexpr uint32_t kFlagEncOneChain = 16;  // separate encode: one MFMA chain per wave, four waves per SIMD
+constexpr uint32_t kFlagsAll = 31;
 
 // One batch of products.  a, b, out: [batch][N][L] u32 coefficient-major.
 // planes: [batch][2][P][N] u32 workspace (operand A planes end up holding the

@@ -31,6 +31,22 @@ import os
 import time
 from pathlib import Path
 
+def _warm_up(run, stream, reps: int, seconds: float = 0.1) -> None:
+    """Untimed runs (at least ``reps``, for at least ``seconds``) before the
+    first candidate is timed: the GPU's clocks and caches are cold at first,
+    which otherwise favours every candidate but the first."""
+    t0 = time.perf_counter()
+    n = 0
+    while n < reps or time.perf_counter() - t0 < seconds:
+        run()
+        n += 1
+        if n % 4 == 0 or n >= reps:
+            stream.synchronize()
+        if n >= 10000:
+            break
+    stream.synchronize()
+
+
 # (chunk, streams); chunk 0 means the whole batch as one chunk.
 DEFAULT_CANDIDATES = ((0, 1), (64, 2), (32, 2), (32, 4), (16, 2), (16, 4), (16, 8), (8, 4), (8, 8))
 
@@ -102,6 +118,7 @@ def tune_encode_tile(engine, a, b, c, batch: int, candidates=None, reps: int = 5
         raise ValueError(f"ByteCRT batch must be in [1, max_batch={engine.max_batch}]")
     stream = stream or torch.cuda.current_stream()
     cands = encode_tile_candidates(engine) if candidates is None else tuple(candidates)
+    _warm_up(lambda: engine.run_stages(31, a=a, b=b, c=c, batch=batch, stream=stream), stream, reps)
     ref = None
     rows = []
     for ts in cands:
@@ -138,6 +155,7 @@ def tune_fusion(engine, a, b, c, batch: int, reps: int = 5, stream=None):
     if batch < 1 or batch > engine.max_batch:
         raise ValueError(f"ByteCRT batch must be in [1, max_batch={engine.max_batch}]")
     stream = stream or torch.cuda.current_stream()
+    _warm_up(lambda: engine.run_stages(31, a=a, b=b, c=c, batch=batch, stream=stream), stream, reps)
     ref = None
     rows = []
     for setting, name in ((FUSE_ALWAYS, "fused"), (FUSE_NEVER, "separate")):
@@ -161,14 +179,16 @@ def tune_fusion(engine, a, b, c, batch: int, reps: int = 5, stream=None):
     return {"best": best, "rows": rows}
 
 
-KERNEL_FLAG_CANDIDATES = tuple(range(8))  # every combination of the three kernel flags
+KERNEL_FLAG_CANDIDATES = tuple(range(32))  # every combination of the five kernel flags
 
 
 def tune_kernel_flags(engine, a, b, c, batch: int, candidates=KERNEL_FLAG_CANDIDATES, reps: int = 3, stream=None):
     """Time the whole product (the four or five kernels, eager launches: no
     graph is captured) under each kernel-flag setting (Engine.set_kernel_flags:
     XCD tile renumbering, the fused encode's MFMA issue priority and its weight
-    staging) for ``batch`` products of device operands a, b (into c).  Checks
+    staging, the CRT built for four or five workgroups per CU with its weight
+    look-ahead, one or two MFMA chains per wave in the separate encode) for
+    ``batch`` products of device operands a, b (into c).  Checks
     every setting's product against the first, bit for bit, and leaves the
     engine on the fastest.  Returns {"best": {...}, "rows": [...]}."""
     import torch
@@ -176,6 +196,7 @@ def tune_kernel_flags(engine, a, b, c, batch: int, candidates=KERNEL_FLAG_CANDID
     if batch < 1 or batch > engine.max_batch:
         raise ValueError(f"ByteCRT batch must be in [1, max_batch={engine.max_batch}]")
     stream = stream or torch.cuda.current_stream()
+    _warm_up(lambda: engine.run_stages(31, a=a, b=b, c=c, batch=batch, stream=stream), stream, reps)
     ref = None
     rows = []
     for flags in candidates:
@@ -214,6 +235,7 @@ def tune_mix(mix, outputs, candidates=MIX_CANDIDATES, reps: int = 5, stream=None
     import torch
 
     stream = stream or torch.cuda.current_stream()
+    _warm_up(lambda: mix.run(stream=stream), stream, reps)
     ref = None
     rows = []
     for chunks, streams, flags in candidates:

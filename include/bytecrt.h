@@ -120,8 +120,13 @@ int bcrt_set_fusion(bcrt_engine* e, uint32_t min_tiles);
  * XCD (and its L2) runs one contiguous eighth; 2 drops the raised issue
  * priority the fused encode gives its MFMA blocks; 4 makes the fused encode's
  * waves load their weight fragments from L2 instead of staging each pair of
- * weight tiles in LDS.  Default 0.  Results do not depend on them.  Applies
- * to later runs and later mixes. */
+ * weight tiles in LDS; 8 runs the CRT built for five workgroups per CU with
+ * its weight fragments loaded one k-step ahead (instead of four per CU, four
+ * k-steps ahead); 16 runs the separate encode (N < 16,384 or batches too
+ * small for the fused kernel) with one MFMA chain per wave at four
+ * workgroups per CU (instead of two chains at three).  Default 0.  Results
+ * do not depend on them.  Applies to later runs and later mixes (bits 8 and
+ * 16 act on single-plan launches only). */
 int bcrt_set_kernel_flags(bcrt_engine* e, uint32_t flags);
 
 /* ---- Mixed batches (products of different N, W, q, one launch per stage) ----

@@ -450,7 +450,9 @@ def test_fherma_mirror_roundtrip(pkg, orc, gu):
 @pytest.mark.parametrize("N,W,B", [(65536, 1747, 2), (16384, 436, 3), (8192, 220, 1)])
 def test_tune_kernel_flags_exact(pkg, orc, gu, tmp_path, N, W, B):
     """Every kernel-flag setting (XCD tile renumbering, the fused encode's
-    MFMA priority and weight staging) gives the same products, bit for bit;
+    MFMA priority and weight staging, the CRT's occupancy and weight
+    look-ahead build, the separate encode's MFMA chains per wave) gives the
+    same products, bit for bit;
     the tuner leaves the engine on the fastest, records it, and a fresh engine
     that applies the table computes the same products (eager and graph).
     Tables built at N = 8,192 exercise the separate encode kernel and the
@@ -465,7 +467,7 @@ def test_tune_kernel_flags_exact(pkg, orc, gu, tmp_path, N, W, B):
     da, db = gu.to_dev(a).view(B, N, -1), gu.to_dev(b).view(B, N, -1)
     out = torch.empty_like(da)
     res = tune.tune_kernel_flags(e, da, db, out, B, reps=1)
-    assert [r["kernel_flags"] for r in res["rows"]] == list(range(8))
+    assert [r["kernel_flags"] for r in res["rows"]] == list(range(32))
     assert e.kernel_flags == res["best"]["kernel_flags"]
     got = gu.from_dev(out).reshape(B, N, e.L)
     np.testing.assert_array_equal(got[0], orc.Plan(N, W, q).product(a[0], b[0]))

@@ -131,4 +131,4 @@ def test_kernel_flag_entries(tune, tmp_path):
     e = FakeEngine(65536, 1747)
     assert tune.apply(e, 128, table) and ("flags", 2) in e.calls and ("fuse", 0) in e.calls
     assert tune.lookup_kernel_flags(32768, 868, 128, table) is None
-    assert tune.KERNEL_FLAG_CANDIDATES == tuple(range(8))
+    assert tune.KERNEL_FLAG_CANDIDATES == tuple(range(32))
