@@ -275,15 +275,14 @@ int create_engine(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W,
         t.kflags = 0;
         t.qt1 = e->plan.qt1;
         t.inv_qt1 = e->plan.inv_qt1;
-        t.primes = upload(*e, e->plan.primes);
         t.pd = upload(*e, e->plan.pd);
         t.zf = upload(*e, e->plan.zf);
         t.zi = upload(*e, e->plan.zi);
         t.scale = upload(*e, e->plan.scale);
+        t.scale_z1 = upload(*e, e->plan.scale_z1);
+        t.zf16 = upload(*e, e->plan.zf16);
         t.enc_prime = upload(*e, e->plan.enc_prime);
         t.wenc = upload(*e, e->plan.wenc);
-        t.enc_off = upload(*e, e->plan.enc_off);
-        t.enc_inv = upload(*e, e->plan.enc_inv);
         t.wcrt = upload(*e, e->plan.wcrt);
         t.corr = upload(*e, e->plan.corr);
         t.invp = upload(*e, e->plan.invp);

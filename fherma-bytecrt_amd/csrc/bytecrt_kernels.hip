@@ -124,11 +124,12 @@ __device__ __forceinline__ void ct_round(double (&v)[16], const uint32_t* __rest
     }
 }
 
-template <int K>
+// (LMIN > 0 stops above the round's lowest LMIN levels.)
+template <int K, int LMIN = 0>
 __device__ __forceinline__ void gs_round(double (&v)[16], const uint32_t* __restrict__ z, uint32_t mult,
                                          uint32_t e0, uint32_t o0, double p, double pinv) {
 #pragma unroll
-    for (int l = K - 1; l >= 0; --l) {
+    for (int l = K - 1; l >= LMIN; --l) {
         const int half = 1 << (K - 1 - l);
 #pragma unroll
         for (int sb = 0; sb < (16 >> K); ++sb) {
@@ -488,10 +489,19 @@ __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __rest
 #pragma unroll
         for (int x = 0; x < 16; ++x) v[x] = word_value(col[x]);
     }
-    gs_round<4>(v, z, 1, 0, 0, p, pinv);
-    const double sc2 = t.scale[j];
+    gs_round<4, 1>(v, z, 1, 0, 0, p, pinv);
+    // Level 0 (twiddle z[1]) with the scale folded in: (x + y) s and
+    // (x - y) (z[1] s), one modular product per output instead of two for
+    // the difference.
+    const double sc2 = t.scale[j], sz = t.scale_z1[j];
 #pragma unroll
-    for (int x = 0; x < 16; ++x) col[tau + T * x] = canon(mul_mod(v[x], sc2, p, pinv), p, pinv);
+    for (int x = 0; x < 8; ++x) {
+        const double d = v[x] - v[x + 8];
+        v[x] = mul_mod(v[x] + v[x + 8], sc2, p, pinv);
+        v[x + 8] = mul_mod(d, sz, p, pinv);
+    }
+#pragma unroll
+    for (int x = 0; x < 16; ++x) col[tau + T * x] = canon(v[x], p, pinv);
 }
 
 // Twiddles of a radix-16 round (K = 4, one sub-block): tw[2^l - 1 + g] is the
@@ -664,13 +674,13 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
 // block, the second-round twiddles into registers during the first round.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kEfS = exchange_stride<256>();
-constexpr uint32_t kEfTw = 15;  // first-round twiddles per prime, z_j[1 .. 15]
+constexpr uint32_t kEfTw = 16;  // first-round twiddle slots per prime: z_j[0 .. 15] as FP64 (z_j[0] unused)
 
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
 
 // Dynamic LDS of the fused kernel: the staging area (later the exchange),
 // the per-prime dot-sum offsets, then one pair of weight tiles.  The current
-// pair's (p, 1/p) and first-round twiddles sit in 1,216 bytes of static LDS.
+// pair's (p, 1/p) and first-round twiddles (FP64) sit in 2,304 bytes of static LDS.
 size_t ef_smem_bytes(const DevTables& t) {
     const size_t stage = size_t(kEncCoefs) * (t.Kenc + 16), exch = size_t(16) * kEfS * sizeof(double);
     // + one pair of weight tiles (64 rows of Kenc + 16 bytes) when Kenc <= 224;
@@ -716,14 +726,14 @@ __device__ __forceinline__ void ct_round_pre(double (&v)[16], const uint32_t (&t
 }
 
 // One forward radix-16 round (column levels 0..3) with its 15 twiddles
-// tw[k] = z[k + 1] (in LDS).
-__device__ __forceinline__ void ct_round_tw15(double (&v)[16], const uint32_t* tw, double p, double pinv) {
+// tw[k] = z[k], k = 1 .. 15, already widened to FP64 (in LDS).
+__device__ __forceinline__ void ct_round_tw15(double (&v)[16], const double* tw, double p, double pinv) {
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
         const int half = 8 >> l;
 #pragma unroll
         for (int g = 0; g < (1 << l); ++g) {
-            const double w = double(tw[(1 << l) + g - 1]);
+            const double w = tw[(1 << l) + g];
 #pragma unroll
             for (int x = 0; x < half; ++x) ct_bfly(v[g * 2 * half + x], v[g * 2 * half + x + half], w, p, pinv);
         }
@@ -734,7 +744,7 @@ template <int KS, int NLOG1, bool STAGE_W>
 __device__ __forceinline__ void encfwd_body(const DevTables& t, const uint32_t* __restrict__ a,
                                             const uint32_t* __restrict__ b, uint32_t* __restrict__ planes,
                                             uint32_t bx, uint32_t op, uint32_t prod, uint8_t* smem,
-                                            double* pbuf, uint32_t* twbuf) {
+                                            double* pbuf, double* twbuf) {
     static_assert(NLOG1 >= 6 && NLOG1 <= 8, "fused encode covers N1 = 64 .. 256");
     constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = 256 / N1, S = exchange_stride<N1>();
     constexpr int K2 = NLOG1 - 4;
@@ -771,17 +781,18 @@ __device__ __forceinline__ void encfwd_body(const DevTables& t, const uint32_t* 
             }
         }
     };
-    // (p, 1/p) and z_j[1 .. 15] of the pair's primes j = 16 grp + jt (clamped
-    // to the last prime), the same way: lanes 0..15 of wave 0 take one 16-byte
-    // (p, 1/p) each, and wave w's lanes the twiddle words 64 w .. 64 w + 63.
+    // (p, 1/p) and the FP64 row z_j[0 .. 15] of the pair's primes j = 16 grp +
+    // jt (clamped to the last prime), the same way: lanes 0..15 of wave 2
+    // take one 16-byte (p, 1/p) each, and lane k of waves 0 and 1 the 16-byte
+    // piece k & 7 of prime k >> 3's row (wave w's 64 pieces land at twbuf +
+    // 128 w).
     auto load_pair_consts = [&](uint32_t grp) {
-        if (wave == 0 && lane < 16)
+        if (wave == 2 && lane < 16)
             __builtin_amdgcn_global_load_lds(t.pd + 2 * min(16 * grp + lane, t.P - 1), (lds_void_ptr)pbuf, 16, 0, 0);
-        const uint32_t k = wave * 64 + lane;
-        if (k < 16 * kEfTw) {
-            const uint32_t jt = k / kEfTw, i = k - jt * kEfTw;
-            __builtin_amdgcn_global_load_lds(t.zf + size_t(min(16 * grp + jt, t.P - 1)) * t.N + 1 + i,
-                                             (lds_void_ptr)(twbuf + wave * 64), 4, 0, 0);
+        if (wave < 2) {
+            const uint32_t k = wave * 64 + lane;
+            __builtin_amdgcn_global_load_lds(t.zf16 + size_t(min(16 * grp + (k >> 3), t.P - 1)) * kEfTw + 2 * (k & 7),
+                                             (lds_void_ptr)(twbuf + wave * 128), 16, 0, 0);
         }
     };
     for (uint32_t j = threadIdx.x; j < t.P; j += kThreads)
@@ -916,7 +927,7 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
     // statically allocated, apart from the dynamic area, so the compiler sees
     // that their loads do not alias the exchange and the weight tiles
     __shared__ __align__(16) double pbuf[32];              // [16][2]: p, 1/p of the pair's primes
-    __shared__ __align__(16) uint32_t twbuf[16 * kEfTw];  // [16][15]: z_j[1 .. 15]
+    __shared__ __align__(16) double twbuf[16 * kEfTw];    // [16][16]: z_j[0 .. 15]
     if (KS <= 7 && !(t.kflags & kFlagWeightsL2))
         encfwd_body<KS, NLOG1, KS <= 7>(t, a, b, planes, bx, op, prod, smem, pbuf, twbuf);
     else
@@ -1781,7 +1792,7 @@ hipError_t launch_product_groups(const GroupDesc* groups, uint32_t ngroups, hipS
 hipError_t configure_kernels() {
     // Every kernel's dynamic LDS stays within the 64 KiB default (the largest,
     // the fused encode at Kenc = 224 with its staged weight tiles, takes 50 KB
-    // besides 1.2 KB of static LDS; the CRT at most 35.8 KB), so no attribute
+    // besides 2.3 KB of static LDS; the CRT at most 35.8 KB), so no attribute
     // needs raising.
     static_assert(size_t(kEncCoefs) * (256 + 16) + sizeof(EncPrime) * kMaxPrimesCrt <= 65536, "encode LDS");
     static_assert(size_t(16) * kEfS * sizeof(double) + sizeof(double) * kMaxPrimesCrt + 64 * (224 + 16) <= 65536,

@@ -15,15 +15,14 @@ struct DevTables {
     uint32_t kflags;               // kernel-level choices (kFlag*), 0 = the measured defaults
     uint64_t qt1;                  // (q >> qshift) + 1
     double inv_qt1;                // 1 / qt1
-    const uint32_t* primes;        // [P]
     const double* pd;              // [P][2]: p, 1/p
     const uint32_t* zf;            // [P][N]: psi^brv(k) mod p
     const uint32_t* zi;            // [P][N]: psi^-brv(k) mod p
     const double* scale;           // [P]: N^-1 (P/p)^-1 mod p
+    const double* scale_z1;        // [P]: scale psi^-brv(1) mod p
+    const double* zf16;            // [P][16]: psi^brv(k) mod p, k < 16
     const void* enc_prime;         // [P] x 24 bytes: off (f64), 1/p rounded down (f64), p, off mod 2^32
     const int8_t* wenc;            // [PTenc][32][Kenc]
-    const uint64_t* enc_off;       // [P]
-    const double* enc_inv;         // [P]: a double <= 1/p
     const int8_t* wcrt;            // [Mcrt][Kcrt]
     const int64_t* corr;           // [Mcrt/4]
     const double* invp;            // [P]

@@ -236,6 +236,8 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
     pl.crt_inverse.resize(P);
     pl.pd.resize(2 * P);
     pl.scale.resize(P);
+    pl.scale_z1.resize(P);
+    pl.zf16.resize(size_t(P) * 16);
     pl.invp.resize(P);
     pl.enc_off.resize(P);
     pl.enc_inv.resize(P);
@@ -286,6 +288,8 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
             zf[k] = pw[e];
             zi[k] = pwi[e];
         }
+        pl.scale_z1[j] = double(mulm(uint32_t(pl.scale[j]), zi[1], p));
+        for (uint32_t k = 0; k < 16; ++k) pl.zf16[size_t(j) * 16 + k] = double(zf[k]);
 
         // Encode weights: r_k == 2^(8k) mod p in (-p/2, p/2], four balanced digits.
         // Row (j % 8) * 4 + d of tile j / 8; column k.

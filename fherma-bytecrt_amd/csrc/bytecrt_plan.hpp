@@ -47,6 +47,8 @@ struct HostPlan {
     std::vector<uint32_t> weights;      // (P+1) x L: P_j mod q, last row (-P) mod q
     std::vector<double> pd;             // P x 2: (p, 1/p) for the FP64 modular products
     std::vector<double> scale;          // P: N^-1 * crt_inverse mod p
+    std::vector<double> scale_z1;       // P: scale * psi^-brv(1) mod p (last inverse level, scale folded in)
+    std::vector<double> zf16;           // P x 16: psi^brv(k) mod p for k < 16 (first forward column round)
     std::vector<uint32_t> zf, zi;       // P x N: psi^{+-brv(k)} mod p
     std::vector<int8_t> wenc;           // PTenc x 32 x Kenc
     std::vector<uint64_t> enc_off;      // P
