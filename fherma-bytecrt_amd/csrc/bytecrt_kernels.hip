@@ -50,7 +50,7 @@ constexpr uint32_t kMaxPrimesCrt = 128;
 //   spacing is 1); r = (h - q p) + l.  h - q p is an integer of magnitude
 //   below 2^31 + 2^18, so the FMA computes it exactly, and the final sum is
 //   exact too.  Hence r == x w (mod p) exactly, with |r| <= p/2 + 2^-12 p.
-// canon(x): |x| <= 2^40 -> the representative in [0, p).
+// canon_product(r): a mul_mod result -> the representative in [0, p).
 // balanced(x): |x| <= 2^40 -> a representative with |r| <= p/2 + 1 (int32).
 // Forward butterflies grow |x| by at most 0.51 p per level, inverse (GS) sums
 // at most double it: eight levels from |x| < p stay below 2^39.
@@ -69,12 +69,8 @@ __device__ __forceinline__ double mul_mod(double x, double w, double p, double p
     return fma(-q, p, h) + l;
 }
 
-__device__ __forceinline__ uint32_t canon(double x, double p, double pinv) {
-    const double q = fma(x, pinv, kRound) - kRound;
-    double r = fma(-q, p, x);  // exact, |r| <= p/2 + 1
-    r = r < 0.0 ? r + p : r;
-    return uint32_t(r);
-}
+// |r| <= p/2 + 2^-12 p < p for a mul_mod result: one conditional add of p.
+__device__ __forceinline__ uint32_t canon_product(double r, double p) { return uint32_t(r < 0.0 ? r + p : r); }
 
 __device__ __forceinline__ int32_t balanced(double x, double p, double pinv) {
     const double q = fma(x, pinv, kRound) - kRound;
@@ -501,7 +497,7 @@ __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __rest
         v[x + 8] = mul_mod(d, sz, p, pinv);
     }
 #pragma unroll
-    for (int x = 0; x < 16; ++x) col[tau + T * x] = canon(v[x], p, pinv);
+    for (int x = 0; x < 16; ++x) col[tau + T * x] = canon_product(v[x], p);
 }
 
 // Twiddles of a radix-16 round (K = 4, one sub-block): tw[2^l - 1 + g] is the

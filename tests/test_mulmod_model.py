@@ -53,3 +53,24 @@ def test_mul_mod_exact_up_to_2_48():
             for w_int in (p_int - 1, p_int // 2, 1):
                 r = mul_mod(float(x_int), float(w_int), p, pinv)
                 assert (int(r) - x_int * w_int) % p_int == 0 and abs(r) <= 0.63 * p_int
+
+
+def canon_product(r: float, p: float) -> int:
+    return int(r + p if r < 0.0 else r)
+
+
+def test_canon_product_of_mul_mod():
+    """The inverse columns write canon_product(mul_mod(x, s)): one conditional
+    add of p, with no quotient step.  A mul_mod result is below p in
+    magnitude (p/2 + 2^-12 p for |x| <= 2^40, the inverse columns' range,
+    0.63 p up to 2^48), so the add gives the representative in [0, p)."""
+    rng = np.random.default_rng(40)
+    primes = [2147352577, 1975386113, (1 << 30) + 3 * 2 ** 17 + 1]
+    for p_int in primes:
+        p, pinv = float(p_int), 1.0 / float(p_int)
+        xs = [int(v) for v in rng.integers(-(1 << 40) + 1, 1 << 40, size=300)]
+        xs += [(1 << 40) - 1, -(1 << 40) + 1, 0, 1, -1, (1 << 47) + 7, -(1 << 47) - 7]
+        for x_int in xs:
+            for w_int in (int(rng.integers(0, p_int)), p_int - 1, 1, p_int // 2):
+                c = canon_product(mul_mod(float(x_int), float(w_int), p, pinv), p)
+                assert 0 <= c < p_int and c == (x_int * w_int) % p_int, (p_int, x_int, w_int)
