@@ -35,6 +35,7 @@ EXPORTED = (
     "bcrt_mix_chunks",
     "bcrt_server_create", "bcrt_server_submit", "bcrt_server_wait", "bcrt_server_stats",
     "bcrt_server_last_error", "bcrt_server_destroy", "bcrt_run_device_noalloc", "bcrt_set_kernel_flags",
+    "bcrt_near_power_gap",
 )
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -89,6 +90,8 @@ def load_library():
     lib.bcrt_set_fusion.argtypes = [vp, ctypes.c_uint32]
     lib.bcrt_set_kernel_flags.restype = ctypes.c_int
     lib.bcrt_set_kernel_flags.argtypes = [vp, ctypes.c_uint32]
+    lib.bcrt_near_power_gap.restype = ctypes.c_int
+    lib.bcrt_near_power_gap.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]
     lib.bcrt_plan_host.restype = ctypes.c_int
     lib.bcrt_plan_host.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p,
                                    _u32p, _u32p, _u32p, _u32p, ctypes.c_char_p, ctypes.c_size_t]
@@ -249,6 +252,7 @@ class Engine:
         self.fuse_min_tiles = min_tiles
 
     FLAG_NO_XCD, FLAG_NO_PRIO, FLAG_WEIGHTS_L2, FLAG_CRT_WIDE, FLAG_ENC_ONE_CHAIN = 1, 2, 4, 8, 16
+    FLAG_GENERIC_Q = 32
 
     def set_kernel_flags(self, flags: int):
         """Kernel-level choices (bytecrt.h bcrt_set_kernel_flags): FLAG_NO_XCD
@@ -256,7 +260,8 @@ class Engine:
         fused encode, FLAG_WEIGHTS_L2 fused-encode weights from L2,
         FLAG_CRT_WIDE the CRT built for five workgroups per CU (weights one
         k-step ahead), FLAG_ENC_ONE_CHAIN the separate encode with one MFMA
-        chain per wave at four workgroups per CU."""
+        chain per wave at four workgroups per CU, FLAG_GENERIC_Q the CRT's
+        generic-q reduction also for a near-power q = 2^W - gap."""
         self._check(load_library().bcrt_set_kernel_flags(self._h, flags))
         self.kernel_flags = flags
 
@@ -414,6 +419,18 @@ def plan_host(N: int, W: int, q: int) -> dict:
     out = {k: int(v) for k, v in zip(keys, geo)}
     out.update(primes=primes, psi=psi, crt_inverse=crt, weights=weights)
     return out
+
+
+def near_power_gap(W: int, q: int) -> int:
+    """2^W - q when below 2^32 (the CRT then takes its near-power
+    reduction), else 0."""
+    L = (W + 31) // 32
+    gap = ctypes.c_uint32(0)
+    qw = limbs_of(q, L)
+    rc = load_library().bcrt_near_power_gap(W, L, qw.ctypes.data_as(_u32p), ctypes.byref(gap))
+    if rc != BCRT_OK:
+        _raise(rc, "ByteCRT near_power_gap: bad arguments")
+    return int(gap.value)
 
 
 def mfma_probe(a, b, c):

@@ -273,6 +273,7 @@ int create_engine(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W,
         t.enc_ts_log = std::min(5u, g.n1);
         t.fuse_min_tiles = bcrt::kFusedMinTiles;
         t.kflags = 0;
+        t.gap = e->plan.gap;
         t.qt1 = e->plan.qt1;
         t.inv_qt1 = e->plan.inv_qt1;
         t.pd = upload(*e, e->plan.pd);
@@ -496,6 +497,12 @@ extern "C" int bcrt_plan_host(uint32_t N, uint32_t W, uint32_t L, const uint32_t
     }
 }
 
+extern "C" int bcrt_near_power_gap(uint32_t W, uint32_t L, const uint32_t* q, uint32_t* gap) {
+    if (!q || !gap) return BCRT_EINVAL;
+    *gap = bcrt::near_power_gap(W, L, q);
+    return BCRT_OK;
+}
+
 extern "C" int bcrt_set_pipeline(bcrt_engine* e, uint32_t chunk, uint32_t streams) {
     if (!e) return BCRT_EINVAL;
     return guarded(e, [&] {
@@ -521,7 +528,7 @@ extern "C" int bcrt_set_encode_tile(bcrt_engine* e, uint32_t ts_log) {
 extern "C" int bcrt_set_kernel_flags(bcrt_engine* e, uint32_t flags) {
     if (!e) return BCRT_EINVAL;
     return guarded(e, [&] {
-        if (flags > bcrt::kFlagsAll) throw std::invalid_argument("ByteCRT kernel flags must be within 0..31");
+        if (flags > bcrt::kFlagsAll) throw std::invalid_argument("ByteCRT kernel flags must be within 0..63");
         drop_graph(e);  // captured graphs hold the tables by value
         e->tables.kflags = flags;
     });

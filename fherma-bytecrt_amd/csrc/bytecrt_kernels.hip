@@ -948,6 +948,9 @@ __device__ __forceinline__ void encfwd_wg(const DevTables& t, const uint32_t* __
 //   t = s_i - Q q_i + carry,  carry = floor(t / 2^32),  digit = t - carry 2^32
 // (|t| < 2^52) yields z - Q q; a wrong estimate (z / q within 2^-40 of an
 // integer) leaves it in [-q, 2q), which one rare add or subtract of q fixes.
+// For q = 2^W - gap (0 < gap < 2^32, L >= 3) the pass uses z - Q q = z +
+// Q gap - Q 2^W instead: Q < 2^18, so Q gap and Q 2^W are a few exact limbs
+// added to limbs 0, 1, L-1 and the top, and the steps need no limbs of q.
 //
 // One LDS region serves three phases: the term rows (GEMM operand), then the
 // int64 limb sums [L+1][64] (written once every wave has left the GEMM; the
@@ -1120,34 +1123,60 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
         // written only after every lane has read limb sums 0..i.
         const uint32_t c = threadIdx.x;
         uint32_t* d = dig + c;  // digit i at d[i * kCrtDigitStride]
-        const double* s = lo + c;  // limb sum i at s[i * kCrtCoefs]
+        double* s = lo + c;     // limb sum i at s[i * kCrtCoefs]
         // Q from the top four limb sums and limbs of q (units of 2^(32 (L-3)))
         const uint32_t b = L - 3;
         const double zt = ((s[L * kCrtCoefs] * 4294967296.0 + s[(L - 1) * kCrtCoefs]) * 4294967296.0 +
                            s[(L - 2) * kCrtCoefs]) * 4294967296.0 + s[b * kCrtCoefs];
         const double qe = (double(t.q[L - 1]) * 4294967296.0 + double(t.q[L - 2])) * 4294967296.0 + double(t.q[b]);
         const double Q = fmax(floor(zt / qe), 0.0);
-        const uint32_t qv = t.q[min(c, L - 1)];  // lane i holds q_i (L <= 64)
-        double carry = 0.0;
-        for (uint32_t i0 = 0; i0 < L; i0 += 8) {
-            // eight limb sums in flight (index clamped to L: every read in bounds
-            // and not yet overwritten by a digit)
-            double sv[8];
+        double carry = 0.0, top = s[L * kCrtCoefs];
+        if (t.gap && !(t.kflags & kFlagGenericQ)) {
+            // near-power q: Q gap and Q 2^r (r = W - 32 (L-1), the top limb's
+            // bits) are below 2^50, exact, and split exactly into 32-bit halves
+            const double qg = Q * double(t.gap), g1 = floor(qg * 0x1p-32), g0 = fma(-g1, 4294967296.0, qg);
+            const double h = ldexp(Q, int(t.W - 32 * (L - 1))), h1 = floor(h * 0x1p-32);
+            const double h0 = fma(-h1, 4294967296.0, h);
+            s[0] += g0;
+            s[kCrtCoefs] += g1;
+            s[(L - 1) * kCrtCoefs] -= h0;
+            top -= h1;
+            for (uint32_t i0 = 0; i0 < L; i0 += 8) {
+                double sv[8];
 #pragma unroll
-            for (uint32_t k = 0; k < 8; ++k) sv[k] = s[min(i0 + k, L) * kCrtCoefs];
+                for (uint32_t k = 0; k < 8; ++k) sv[k] = s[min(i0 + k, L) * kCrtCoefs];
 #pragma unroll
-            for (uint32_t k = 0; k < 8; ++k) {
-                const uint32_t i = i0 + k;
-                if (i < L) {
-                    const double qi = double(uint32_t(__builtin_amdgcn_readlane(int(qv), int(i))));
-                    const double tt = fma(-Q, qi, sv[k]) + carry;
-                    carry = floor(tt * 0x1p-32);
-                    d[i * kCrtDigitStride] = uint32_t(fma(-carry, 4294967296.0, tt));
+                for (uint32_t k = 0; k < 8; ++k) {
+                    const uint32_t i = i0 + k;
+                    if (i < L) {
+                        const double tt = sv[k] + carry;
+                        carry = floor(tt * 0x1p-32);
+                        d[i * kCrtDigitStride] = uint32_t(fma(-carry, 4294967296.0, tt));
+                    }
+                }
+            }
+        } else {
+            const uint32_t qv = t.q[min(c, L - 1)];  // lane i holds q_i (L <= 64)
+            for (uint32_t i0 = 0; i0 < L; i0 += 8) {
+                // eight limb sums in flight (index clamped to L: every read in
+                // bounds and not yet overwritten by a digit)
+                double sv[8];
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k) sv[k] = s[min(i0 + k, L) * kCrtCoefs];
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k) {
+                    const uint32_t i = i0 + k;
+                    if (i < L) {
+                        const double qi = double(uint32_t(__builtin_amdgcn_readlane(int(qv), int(i))));
+                        const double tt = fma(-Q, qi, sv[k]) + carry;
+                        carry = floor(tt * 0x1p-32);
+                        d[i * kCrtDigitStride] = uint32_t(fma(-carry, 4294967296.0, tt));
+                    }
                 }
             }
         }
         // r = z - Q q = sum d_i 2^(32 i) + top 2^(32 L), -q <= r < 2q
-        const double top = s[L * kCrtCoefs] + carry;
+        top += carry;
         bool sub = top > 0.0, add = top < 0.0;
         if (top == 0.0) {
             sub = true;  // r == q unless a limb differs

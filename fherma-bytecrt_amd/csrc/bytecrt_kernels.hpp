@@ -13,6 +13,7 @@ struct DevTables {
     uint32_t enc_ts_log;           // encode tile: 2^enc_ts_log consecutive s by 128 >> enc_ts_log r
     uint32_t fuse_min_tiles;       // host: fused encode + forward columns from this many workgroups
     uint32_t kflags;               // kernel-level choices (kFlag*), 0 = the measured defaults
+    uint32_t gap;                  // q = 2^W - gap, 0 < gap < 2^32, L >= 3 (near-power CRT reduction); else 0
     uint64_t qt1;                  // (q >> qshift) + 1
     double inv_qt1;                // 1 / qt1
     const double* pd;              // [P][2]: p, 1/p
@@ -36,7 +37,8 @@ constexpr uint32_t kFlagNoPrio = 2;     // fused encode: no raised issue priorit
 constexpr uint32_t kFlagWeightsL2 = 4;  // fused encode: weight fragments from L2, not staged in LDS
 constexpr uint32_t kFlagCrtWide = 8;    // CRT built for five workgroups per CU, weights one k-step ahead
 constexpr uint32_t kFlagEncOneChain = 16;  // separate encode: one MFMA chain per wave, four waves per SIMD
-constexpr uint32_t kFlagsAll = 31;
+constexpr uint32_t kFlagGenericQ = 32;     // CRT: the generic-q reduction also for q = 2^W - gap
+constexpr uint32_t kFlagsAll = 63;
 
 // One batch of products.  a, b, out: [batch][N][L] u32 coefficient-major.
 // planes: [batch][2][P][N] u32 workspace (operand A planes end up holding the

@@ -195,6 +195,16 @@ uint32_t select_primes(uint32_t N, uint32_t W, std::vector<uint32_t>* primes, Na
 
 uint32_t plan_prime_count(uint32_t N, uint32_t W) { return select_primes(N, W, nullptr, nullptr); }
 
+uint32_t near_power_gap(uint32_t W, uint32_t L, const uint32_t* qw) {
+    if (!qw || L < 3 || W < 64 || L != (W + 31) / 32) return 0;
+    Nat q(qw, qw + L);
+    trim(q);
+    if (bits(q) != W) return 0;
+    Nat gap = shl(Nat{1}, W);
+    sub_in(gap, q);
+    return bits(gap) <= 32 ? gap[0] : 0;  // gap >= 1: q < 2^W
+}
+
 HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
     if (N < 4096 || N > 65536 || (N & (N - 1)))
         throw std::invalid_argument("ByteCRT plan: N must be a power of two in [4096, 65536]");
@@ -369,6 +379,7 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
     }
     pl.qt1 = qt + 1;
     pl.inv_qt1 = 1.0 / double(qt + 1);
+    pl.gap = near_power_gap(W, L, qw);
     return pl;
 }
 

@@ -60,6 +60,7 @@ struct HostPlan {
     uint32_t qshift = 0;                // W - 32
     uint64_t qt1 = 0;                   // (q >> qshift) + 1
     double inv_qt1 = 0;                 // 1 / qt1
+    uint32_t gap = 0;                   // 2^W - q when below 2^32 and L >= 3 (near-power CRT reduction), else 0
 };
 
 // Throws std::invalid_argument (bad geometry or q) or std::runtime_error (no plan).
@@ -67,5 +68,10 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q);
 
 // Number of primes the plan would use for (N, W) (no q needed).
 uint32_t plan_prime_count(uint32_t N, uint32_t W);
+
+// gap when q (L limbs, bit length W, L >= 3) is 2^W - gap with gap < 2^32 --
+// the form the CRT's near-power reduction takes, and the only one the
+// reference accepts (rns-plan.hpp:186-197); otherwise 0.
+uint32_t near_power_gap(uint32_t W, uint32_t L, const uint32_t* q);
 
 }  // namespace bcrt

@@ -179,7 +179,9 @@ def tune_fusion(engine, a, b, c, batch: int, reps: int = 5, stream=None):
     return {"best": best, "rows": rows}
 
 
-KERNEL_FLAG_CANDIDATES = tuple(range(32))  # every combination of the five kernel flags
+# every combination of the five performance flags (32, the CRT's generic-q
+# reduction for near-power moduli, is a switch for tests)
+KERNEL_FLAG_CANDIDATES = tuple(range(32))
 
 
 def tune_kernel_flags(engine, a, b, c, batch: int, candidates=KERNEL_FLAG_CANDIDATES, reps: int = 3, stream=None):

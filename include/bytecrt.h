@@ -124,9 +124,12 @@ int bcrt_set_fusion(bcrt_engine* e, uint32_t min_tiles);
  * its weight fragments loaded one k-step ahead (instead of four per CU, four
  * k-steps ahead); 16 runs the separate encode (N < 16,384 or batches too
  * small for the fused kernel) with one MFMA chain per wave at four
- * workgroups per CU (instead of two chains at three).  Default 0.  Results
- * do not depend on them.  Applies to later runs and later mixes (bits 8 and
- * 16 act on single-plan launches only). */
+ * workgroups per CU (instead of two chains at three); 32 makes the CRT reduce
+ * a near-power modulus q = 2^W - gap (0 < gap < 2^32) by the generic-q steps
+ * instead of its near-power steps (g1-crt.cuh:15-68; the tuner leaves this bit
+ * to tests, which compare the two).  Default 0.  Results do not depend on
+ * them.  Applies to later runs and later mixes (bits 8 and 16 act on
+ * single-plan launches only). */
 int bcrt_set_kernel_flags(bcrt_engine* e, uint32_t flags);
 
 /* ---- Mixed batches (products of different N, W, q, one launch per stage) ----
@@ -230,6 +233,13 @@ int bcrt_plan_tables(const bcrt_engine* e, uint32_t* primes, uint32_t* psi, uint
 int bcrt_plan_host(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, uint32_t* geometry,
                    uint32_t* primes, uint32_t* psi, uint32_t* crt_inverse, uint32_t* weights,
                    char* err, size_t errlen);
+
+/* *gap = 2^W - q when that is below 2^32 (q of L >= 3 limbs and bit length
+ * W), else 0.  A plan for such a q reduces the CRT sum by the near-power
+ * steps (q = 2^W - gap is the only modulus form make_rns_plan accepts,
+ * rns-plan.hpp:186-197, reduced by reduce_near_power, g1-crt.cuh:45-68);
+ * any other q takes the generic-q steps. */
+int bcrt_near_power_gap(uint32_t W, uint32_t L, const uint32_t* q, uint32_t* gap);
 
 /* Stage-level access for parity tests and profiling.  Residue planes
  * ([batch][2][P][N] u32, column-major positions) live in d_planes (NULL = the

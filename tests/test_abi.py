@@ -48,6 +48,20 @@ def test_host_plan_matches_oracle(pkg, orc, N, W, seed):
     assert mine["Kcrt"] == ref.kpad  # 4(P+1) padded to 32, as the reference's Kpad
 
 
+def test_near_power_gap(pkg):
+    """The CRT takes its near-power reduction exactly for q = 2^W - gap,
+    0 < gap < 2^32 (the reference's only modulus form)."""
+    gap_of = pkg.bytecrt.near_power_gap
+    for W in (96, 220, 868, 1747):
+        for gap in (1, 2, 65535, (1 << 31) + 1, (1 << 32) - 1):
+            assert gap_of(W, (1 << W) - gap) == gap
+        assert gap_of(W, (1 << W) - (1 << 32)) == 0
+        assert gap_of(W, (1 << W) - (1 << 32) - 1) == 0
+        assert gap_of(W, pkg.datagen.modulus(W, 3)) == 0  # random: top 32 bits not all ones
+        assert gap_of(W, (1 << (W - 1)) + 1) == 0
+    assert gap_of(64, (1 << 64) - 1) == 0  # two limbs: the generic steps
+
+
 def test_survey_prime_counts(pkg):
     # N 8192..65536 with W 220/436/868/1747 -> 15/29/57/114 primes.
     for N, W, P in ((8192, 220, 15), (16384, 436, 29), (32768, 868, 57), (65536, 1747, 114)):

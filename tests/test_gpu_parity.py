@@ -83,14 +83,50 @@ def test_crt_stage_bcrt01(pkg, orc, gu, path):
     pos = np.zeros((2, e.P, N), dtype=np.uint32)
     pos[0, :, :fx["m"]] = fx["terms"].T  # plane positions f = 0..15
     planes = gu.to_dev(pos.reshape(-1))
-    out = gu.empty_dev(N * e.L)
-    e.run_stages(pkg.bytecrt.STAGE_CRT, c=out, planes=planes)
-    got = gu.from_dev(out).reshape(N, e.L)
     nat = gu.natural_of_position(N, e.N1)
-    np.testing.assert_array_equal(got[nat[:fx["m"]]], fx["golden"])
     rest = np.ones(N, dtype=bool)
     rest[nat[:fx["m"]]] = False
-    assert not got[rest].any()
+    # every fixture modulus is 2^W - gap: the near-power reduction, then the
+    # generic-q steps on the same terms
+    for flags in (0, e.FLAG_GENERIC_Q):
+        e.set_kernel_flags(flags)
+        out = gu.empty_dev(N * e.L)
+        e.run_stages(pkg.bytecrt.STAGE_CRT, c=out, planes=planes)
+        got = gu.from_dev(out).reshape(N, e.L)
+        np.testing.assert_array_equal(got[nat[:fx["m"]]], fx["golden"])
+        assert not got[rest].any()
+
+
+@pytest.mark.parametrize("N,W,gap", [(8192, 220, 1), (8192, 220, (1 << 32) - 1), (32768, 868, 2 * 32768 - 1),
+                                     (65536, 1747, 0x9E3779B9)])
+def test_crt_stage_near_power_against_generic(pkg, orc, gu, N, W, gap):
+    """CRT of random canonical terms t_j < p_j under q = 2^W - gap, with the
+    extremes (all t_j = 0, all t_j = p_j - 1, one term nonzero) planted: the
+    near-power reduction equals the generic-q steps everywhere, and the
+    oracle's CRT on sampled positions."""
+    q = (1 << W) - gap
+    e = pkg.Engine(N, W, q)
+    primes = e.plan_tables()["primes"].astype(np.uint64)
+    rng = np.random.default_rng(gap % 1000003)
+    terms = (rng.integers(0, 1 << 62, size=(N, e.P), dtype=np.uint64) % primes).astype(np.uint32)
+    terms[0] = 0
+    terms[1] = (primes - 1).astype(np.uint32)
+    terms[2] = 0
+    terms[2, e.P // 2] = 1
+    pos = np.zeros((2, e.P, N), dtype=np.uint32)
+    pos[0] = terms.T  # plane position f holds row f
+    planes = gu.to_dev(pos.reshape(-1))
+    outs = []
+    for flags in (0, e.FLAG_GENERIC_Q):
+        e.set_kernel_flags(flags)
+        out = gu.empty_dev(N * e.L)
+        e.run_stages(pkg.bytecrt.STAGE_CRT, c=out, planes=planes)
+        outs.append(gu.from_dev(out).reshape(N, e.L))
+    np.testing.assert_array_equal(outs[0], outs[1])
+    nat = gu.natural_of_position(N, e.N1)
+    sample = np.concatenate([np.arange(3), rng.choice(np.arange(3, N), size=253, replace=False)])
+    want = orc.Plan(N, W, q).crt_terms(terms[sample])
+    np.testing.assert_array_equal(outs[0][nat[sample]], want)
 
 
 def product_case(pkg, orc, gu, N, W, q, seed, edge_fraction=0.0):
