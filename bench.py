@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
     ap.add_argument("--only-stage", choices=["encode", "fwd_cols", "encode_fwd", "middle", "inv_cols", "crt", "product",
                                              "hbm_copy", "hbm_read", "hbm_write", "bfly_fp64", "bfly_shoup32",
-                                             "bfly_montgomery32"],
+                                             "bfly_montgomery32", "overlap"],
                     help="profiling mode: run one stage (or the whole product, or one probe) --reps times")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--sweep", action="store_true",
@@ -79,6 +79,18 @@ def profile_mode(args, torch):
             probes.hbm_stream(mode, src, dst, nbytes)
         torch.cuda.synchronize()
         print(json.dumps({"hbm_probe": probes.MODES[mode], "bytes": nbytes, "reps": args.reps}), flush=True)
+        return
+    if args.only_stage == "overlap":
+        # every (mode, kind) of the MFMA/VALU overlap probe, reps launches each
+        words = int(probes.load_library().bcrtp_issue_overlap_words())
+        out = torch.zeros(words, dtype=torch.int32, device="cuda")
+        for kind in probes.KINDS:
+            for mode in (1, 2, 3):
+                for _ in range(args.reps):
+                    probes.issue_overlap(mode, kind, 4000, 16000, out)
+        torch.cuda.synchronize()
+        print(json.dumps({"overlap_probe": list(probes.KINDS.values()), "mfma_iters": 4000, "valu_iters": 16000,
+                          "reps": args.reps}), flush=True)
         return
     q = pkg.datagen.modulus(W, 2026)
     if args.only_stage.startswith("bfly_"):
@@ -599,7 +611,8 @@ def main():
     pt = eng.plan_tables()
     p0, psi0 = int(pt["primes"][0]), int(pt["psi"][0])
     ceilings = {"hbm": pkg.probes.measure_hbm(torch),
-                "butterflies": pkg.probes.measure_butterflies(torch, p0, probe_tw15(p0, psi0))}
+                "butterflies": pkg.probes.measure_butterflies(torch, p0, probe_tw15(p0, psi0)),
+                "issue_overlap": pkg.probes.measure_overlap(torch)}
     if roof["bound"] == "hbm":
         roof["practical_peak"] = ceilings["hbm"]["copy_GBps"]
         roof["practical_frac"] = round(roof["achieved"] / roof["practical_peak"], 4)

@@ -1,5 +1,6 @@
 // Measurement probes for gfx950 (include/bytecrt_probes.h): HBM stream
-// kernels and butterfly kernels for three exact modular multiplications.
+// kernels, butterfly kernels for three exact modular multiplications, and the
+// overlap of MFMA and VALU work from different waves of one SIMD.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -160,7 +161,105 @@ __global__ __launch_bounds__(kThreads) void k_bfly(uint32_t* __restrict__ out, c
     }
 }
 
+// ---- MFMA and VALU issue overlap -------------------------------------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+constexpr uint32_t kOverlapThreads = 512;  // waves 0..3 MFMA, 4..7 VALU: one of each per SIMD
+constexpr uint32_t kOverlapGridPerCu = 2;
+
+// The MFMA waves run four independent v_mfma_i32_32x32x32_i8 chains
+// (mfma_iters x 4 MFMAs); the VALU waves run eight independent chains of one
+// operation (valu_iters x 8): kind 0 FP64 fma, 1 FP32 fma, 2 int32 24-bit
+// multiply-add.  mode bit 1 runs the MFMA waves, bit 2 the VALU waves; the
+// others exit at once.  Each running thread writes one word of its results.
+// (MODE and KIND are template parameters so that a kernel trace names each
+// combination.)
+template <int MODE, int KIND>
+__global__ __launch_bounds__(kOverlapThreads) void k_overlap(uint32_t mfma_iters, uint32_t valu_iters,
+                                                             int32_t* __restrict__ out) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t slot = size_t(blockIdx.x) * kOverlapThreads + threadIdx.x;
+    if (wave < 4) {
+        if constexpr (!(MODE & 1)) return;
+        const v4i a = {int(lane), int(lane) + 1, int(lane) + 2, int(lane) + 3};
+        const v4i b = {int(lane) * 3, int(lane), 7, 9};
+        v16i c0 = {}, c1 = {}, c2 = {}, c3 = {};
+        for (uint32_t i = 0; i < mfma_iters; ++i) {
+            c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c1, 0, 0, 0);
+            c2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c2, 0, 0, 0);
+            c3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c3, 0, 0, 0);
+        }
+        int32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sum += c0[k] + c1[k] + c2[k] + c3[k];
+        out[slot] = sum;
+        return;
+    }
+    if constexpr (!(MODE & 2)) return;
+    if constexpr (KIND == 0) {
+        double x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = double(lane + k);
+        for (uint32_t i = 0; i < valu_iters; ++i)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) x[k] = fma(x[k], 1.0000001, 0.5);
+        double sum = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sum += x[k];
+        out[slot] = int32_t(__double2loint(sum) ^ __double2hiint(sum));
+    } else if constexpr (KIND == 1) {
+        float x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = float(lane + k);
+        for (uint32_t i = 0; i < valu_iters; ++i)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) x[k] = fmaf(x[k], 1.0000001f, 0.5f);
+        float sum = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sum += x[k];
+        out[slot] = __float_as_int(sum);
+    } else {
+        uint32_t x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = lane + k;
+        for (uint32_t i = 0; i < valu_iters; ++i)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) x[k] = __umul24(x[k], 0x9E3779u) + 12345u;
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sum += x[k];
+        out[slot] = int32_t(sum);
+    }
+}
+
 }  // namespace
+
+extern "C" int bcrtp_issue_overlap(int mode, int kind, uint32_t mfma_iters, uint32_t valu_iters, int32_t* d_out,
+                                   size_t out_words, void* stream) {
+    if (mode < 1 || mode > 3 || kind < 0 || kind > 2 || !d_out) return BCRT_EINVAL;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return BCRT_ERUNTIME;
+    const dim3 grid(uint32_t(cus) * kOverlapGridPerCu), blk(kOverlapThreads);
+    if (out_words < size_t(grid.x) * kOverlapThreads) return BCRT_EINVAL;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void (*const kernels[3][3])(uint32_t, uint32_t, int32_t*) = {
+        {k_overlap<1, 0>, k_overlap<1, 1>, k_overlap<1, 2>},
+        {k_overlap<2, 0>, k_overlap<2, 1>, k_overlap<2, 2>},
+        {k_overlap<3, 0>, k_overlap<3, 1>, k_overlap<3, 2>}};
+    hipLaunchKernelGGL(kernels[mode - 1][kind], grid, blk, 0, s, mfma_iters, valu_iters, d_out);
+    return hipGetLastError() == hipSuccess ? BCRT_OK : BCRT_ERUNTIME;
+}
+
+extern "C" size_t bcrtp_issue_overlap_words() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return size_t(cus) * kOverlapGridPerCu * kOverlapThreads;
+}
 
 extern "C" int bcrtp_hbm_stream(int mode, const void* d_src, void* d_dst, size_t bytes, void* stream) {
     if (mode < 0 || mode > 2 || bytes == 0 || bytes % 16 || !d_dst || (mode != 2 && !d_src)) return BCRT_EINVAL;

@@ -15,7 +15,8 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "lib" / "libbytecrt_probes.so"
-EXPORTED = ("bcrtp_hbm_stream", "bcrtp_butterflies")
+EXPORTED = ("bcrtp_hbm_stream", "bcrtp_butterflies", "bcrtp_issue_overlap", "bcrtp_issue_overlap_words")
+KINDS = {0: "fp64_fma", 1: "fp32_fma", 2: "int24_mad"}
 FORMS = {0: "fp64", 1: "shoup32", 2: "montgomery32"}
 MODES = {0: "copy", 1: "read", 2: "write"}
 _LIB = None
@@ -33,6 +34,11 @@ def load_library():
         lib.bcrtp_butterflies.restype = ctypes.c_int
         lib.bcrtp_butterflies.argtypes = [ctypes.c_int, vp, ctypes.c_size_t, ctypes.c_uint32,
                                           ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, vp]
+        lib.bcrtp_issue_overlap.restype = ctypes.c_int
+        lib.bcrtp_issue_overlap.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, vp,
+                                            ctypes.c_size_t, vp]
+        lib.bcrtp_issue_overlap_words.restype = ctypes.c_size_t
+        lib.bcrtp_issue_overlap_words.argtypes = []
         _LIB = lib
     return _LIB
 
@@ -50,6 +56,13 @@ def butterflies(form: int, out, threads: int, p: int, tw15, rounds: int, stream=
                                          stream.cuda_stream if stream is not None else None)
     if rc != 0:
         raise RuntimeError(f"ByteCRT butterfly probe form {form} failed ({rc})")
+
+
+def issue_overlap(mode: int, kind: int, mfma_iters: int, valu_iters: int, out, stream=None):
+    rc = load_library().bcrtp_issue_overlap(mode, kind, mfma_iters, valu_iters, out.data_ptr(), out.numel(),
+                                           stream.cuda_stream if stream is not None else None)
+    if rc != 0:
+        raise RuntimeError(f"ByteCRT overlap probe mode {mode} kind {kind} failed ({rc})")
 
 
 def _timed(torch, stream, fn, reps: int) -> float:
@@ -98,4 +111,31 @@ def measure_butterflies(torch, p: int, tw15, threads: int = 1 << 20, rounds: int
     torch.cuda.synchronize()
     res["forms_agree"] = bool(torch.equal(outs["fp64"], outs["shoup32"]) and
                               torch.equal(outs["fp64"], outs["montgomery32"]))
+    return res
+
+
+def measure_overlap(torch, mfma_iters: int = 4000, valu_iters=None, reps: int = 5) -> dict:
+    """Milliseconds of MFMA waves alone, VALU waves alone and both together on
+    the same SIMDs, per VALU kind, and overlap = (alone sum - both) / smaller
+    alone (1: the smaller hides completely, 0: they take turns).  Checks that
+    running together changes no wave's results."""
+    valu_iters = valu_iters or {0: 16000, 1: 16000, 2: 16000}
+    words = int(load_library().bcrtp_issue_overlap_words())
+    stream = torch.cuda.Stream()
+    res = {"mfma_iters": mfma_iters, "valu_iters": valu_iters}
+    agree = True
+    for kind, name in KINDS.items():
+        outs, ms = {}, {}
+        for mode in (1, 2, 3):
+            out = torch.zeros(words, dtype=torch.int32, device="cuda")
+            ms[mode] = _timed(torch, stream,
+                              lambda: issue_overlap(mode, kind, mfma_iters, valu_iters[kind], out, stream), reps)
+            outs[mode] = out.view(-1, 512)
+        torch.cuda.synchronize()
+        agree &= bool(torch.equal(outs[3][:, :256], outs[1][:, :256]) and
+                      torch.equal(outs[3][:, 256:], outs[2][:, 256:]))
+        small = min(ms[1], ms[2])
+        res[name] = {"mfma_ms": round(ms[1], 4), "valu_ms": round(ms[2], 4), "both_ms": round(ms[3], 4),
+                     "overlap": round((ms[1] + ms[2] - ms[3]) / small, 3)}
+    res["results_agree"] = agree
     return res

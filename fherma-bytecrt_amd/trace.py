@@ -91,8 +91,9 @@ def _rows(path):
 
 
 def short_kernel(name: str) -> str:
-    m = re.search(r"(k_\w+(<\d+>)?)", name)
-    return m.group(1) if m else name.split("(")[0][-60:]
+    """k_name<args> (integer and bool template arguments kept, spaces dropped)."""
+    m = re.search(r"(k_\w+(<[\w, ]+>)?)", name)
+    return m.group(1).replace(" ", "") if m else name.split("(")[0][-60:]
 
 
 def load(d: Path):

@@ -38,6 +38,19 @@ int bcrtp_hbm_stream(int mode, const void* d_src, void* d_dst, size_t bytes, voi
 int bcrtp_butterflies(int form, uint32_t* d_out, size_t threads, uint32_t p, const uint32_t* tw15, uint32_t rounds,
                       void* stream);
 
+/* Overlap of matrix-core and vector work on one SIMD: a grid of two 512-thread
+ * workgroups per CU, whose waves 0..3 (one per SIMD) each run 4 x mfma_iters
+ * v_mfma_i32_32x32x32_i8 (four independent chains) and whose waves 4..7 run
+ * 8 x valu_iters operations of one kind (eight independent chains): kind 0
+ * FP64 fma, 1 FP32 fma, 2 int32 24-bit multiply-add.  mode 1 runs only the
+ * MFMA waves, 2 only the VALU waves, 3 both; a running thread writes one word
+ * d_out[512 * workgroup + thread] (out_words >= bcrtp_issue_overlap_words()).
+ * Timing modes 1, 2 and 3 shows whether the two kinds of work overlap (mode 3
+ * near the larger of 1 and 2) or take turns (near their sum). */
+int bcrtp_issue_overlap(int mode, int kind, uint32_t mfma_iters, uint32_t valu_iters, int32_t* d_out,
+                        size_t out_words, void* stream);
+size_t bcrtp_issue_overlap_words(void);
+
 #ifdef __cplusplus
 }
 #endif

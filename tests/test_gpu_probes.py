@@ -1,7 +1,9 @@
 """GPU checks of the measurement probes (include/bytecrt_probes.h): the HBM
-copy kernel copies exactly, and the three butterfly forms (FP64 as the
+copy kernel copies exactly, the three butterfly forms (FP64 as the
 product kernels compute, 32-bit Shoup, 32-bit Montgomery) produce the same
-residues, equal to a Python-integer model of the same rounds."""
+residues, equal to a Python-integer model of the same rounds, and the
+MFMA/VALU overlap probe computes the same words alone and together."""
+import numpy as np
 import pytest
 import torch
 
@@ -59,3 +61,33 @@ def test_hbm_copy_probe_exact(pkg):
     pkg.probes.hbm_stream(2, None, dst, n * 4)
     pkg.probes.hbm_stream(1, src, dst, n * 4)
     torch.cuda.synchronize()
+
+
+def test_issue_overlap_probe(pkg):
+    pr = pkg.probes
+    words = int(pr.load_library().bcrtp_issue_overlap_words())
+    assert words > 0 and words % 512 == 0
+
+    def run(mode, kind, mi, vi):
+        out = torch.zeros(words, dtype=torch.int32, device="cuda")
+        pr.issue_overlap(mode, kind, mi, vi, out)
+        torch.cuda.synchronize()
+        return out.cpu().numpy().reshape(-1, 512)
+
+    alone_m, alone_v, both = run(1, 2, 50, 60), run(2, 2, 50, 60), run(3, 2, 50, 60)
+    assert (both[:, :256] == alone_m[:, :256]).all() and (both[:, 256:] == alone_v[:, 256:]).all()
+    assert not alone_m[:, 256:].any() and not alone_v[:, :256].any()
+    # int kind: x <- (x mod 2^24) * 0x9E3779 + 12345 mod 2^32, eight chains from lane + k
+    x = (np.arange(256, dtype=np.uint64) % 64)[:, None] + np.arange(8, dtype=np.uint64)
+    for _ in range(60):
+        x = ((x & 0xFFFFFF) * 0x9E3779 + 12345) & 0xFFFFFFFF
+    want = (x.sum(axis=1) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+    assert (alone_v[:, 256:] == want[None, :]).all()
+    # MFMA waves: the sums are linear in the iteration count (mod 2^32)
+    twice = run(1, 2, 100, 60)
+    assert (twice[:, :256] == (alone_m[:, :256].astype(np.int64) * 2).astype(np.int32)).all()
+    for kind in (0, 1):
+        a, b = run(2, kind, 1, 40), run(3, kind, 1, 40)
+        assert (a[:, 256:] == b[:, 256:]).all()
+    with pytest.raises(RuntimeError):
+        pr.issue_overlap(4, 0, 1, 1, torch.zeros(words, dtype=torch.int32, device="cuda"))

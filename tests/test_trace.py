@@ -63,3 +63,12 @@ def test_decode_counters_scaled(tr, tmp_path):
           [["k_crt(x)", "FETCH_SIZE", "10"], ["k_crt(x)", "FETCH_SIZE", "30"], ["k_crt(x)", "WRITE_SIZE", "4"]])
     res = tr.decode(tmp_path, {"FETCH_SIZE": 2.0})
     assert res["counters"]["k_crt"] == {"FETCH_SIZE": 40.0, "WRITE_SIZE": 4.0}
+
+
+def test_short_kernel_names(tr):
+    sk = tr.short_kernel
+    assert sk("void bcrt::(anonymous namespace)::k_middle(bcrt::DevTables, unsigned int*)") == "k_middle"
+    assert sk("void bcrt::(anonymous namespace)::k_crt<15, false>(bcrt::DevTables, unsigned int const*)") == \
+        "k_crt<15,false>"
+    assert sk("void (anonymous namespace)::k_overlap<3, 0>(unsigned int, unsigned int, int*)") == "k_overlap<3,0>"
+    assert sk("void bcrt::(anonymous namespace)::k_inv_cols<8>(bcrt::DevTables, unsigned int*)") == "k_inv_cols<8>"
