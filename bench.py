@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
     ap.add_argument("--only-stage", choices=["encode", "fwd_cols", "encode_fwd", "middle", "inv_cols", "crt", "product",
                                              "hbm_copy", "hbm_read", "hbm_write", "bfly_fp64", "bfly_shoup32",
-                                             "bfly_montgomery32", "overlap"],
+                                             "bfly_montgomery32", "bfly_fp64_tw5", "bfly_fp64_tw5_pairs", "bfly_fp64_fma_product", "overlap"],
                     help="profiling mode: run one stage (or the whole product, or one probe) --reps times")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--sweep", action="store_true",
@@ -96,7 +96,8 @@ def profile_mode(args, torch):
     if args.only_stage.startswith("bfly_"):
         pl = pkg.bytecrt.plan_host(N, W, q)
         p, psi = int(pl["primes"][0]), int(pl["psi"][0])
-        form = {"bfly_fp64": 0, "bfly_shoup32": 1, "bfly_montgomery32": 2}[args.only_stage]
+        form = {"bfly_fp64": 0, "bfly_shoup32": 1, "bfly_montgomery32": 2, "bfly_fp64_tw5": 3,
+                "bfly_fp64_tw5_pairs": 4, "bfly_fp64_fma_product": 5}[args.only_stage]
         threads, rounds = 1 << 20, 64
         out = torch.empty(threads * 16, dtype=torch.int32, device="cuda")
         for _ in range(args.reps):
@@ -617,9 +618,16 @@ def main():
         roof["practical_peak"] = ceilings["hbm"]["copy_GBps"]
         roof["practical_frac"] = round(roof["achieved"] / roof["practical_peak"], 4)
     # NTT butterflies per batch (2 forward + 1 inverse transform per prime,
-    # N/2 log2 N each) at the measured FP64 butterfly rate
-    ntt_bfly = 3 * P * (N // 2) * 16 * B
-    ceilings["ntt_fp64_floor_ms"] = round(ntt_bfly / (ceilings["butterflies"]["fp64_Tbfly_per_s"] * 1e12) * 1e3, 3)
+    # N/2 log2 N each) at the measured rates of the forms the kernels use: the
+    # fused encode's forward column levels (n1 per operand) the six-operation
+    # form with FMA products, the middle and inverse-column levels the
+    # five-operation form (both at four waves per SIMD, the kernels' occupancy)
+    bf = ceilings["butterflies"]
+    n1 = (N // 256).bit_length() - 1
+    cols_bfly = 2 * P * (N // 2) * n1 * B
+    rest_bfly = 3 * P * (N // 2) * 16 * B - cols_bfly
+    ceilings["ntt_fp64_floor_ms"] = round((cols_bfly / bf["fp64_fma_product_4w_Tbfly_per_s"] +
+                                           rest_bfly / bf["fp64_tw5_4w_Tbfly_per_s"]) / 1e12 * 1e3, 3)
     # NTT-only kernels of the product path (the forward column levels run
     # inside the fused encode kernel)
     ntt_ms = dur["middle"] + dur["inv_cols"]

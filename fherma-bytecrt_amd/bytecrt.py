@@ -252,7 +252,7 @@ class Engine:
         self.fuse_min_tiles = min_tiles
 
     FLAG_NO_XCD, FLAG_NO_PRIO, FLAG_WEIGHTS_L2, FLAG_CRT_WIDE, FLAG_ENC_ONE_CHAIN = 1, 2, 4, 8, 16
-    FLAG_GENERIC_Q = 32
+    FLAG_GENERIC_Q, FLAG_MULMOD_TW6 = 32, 64
 
     def set_kernel_flags(self, flags: int):
         """Kernel-level choices (bytecrt.h bcrt_set_kernel_flags): FLAG_NO_XCD
@@ -261,7 +261,8 @@ class Engine:
         FLAG_CRT_WIDE the CRT built for five workgroups per CU (weights one
         k-step ahead), FLAG_ENC_ONE_CHAIN the separate encode with one MFMA
         chain per wave at four workgroups per CU, FLAG_GENERIC_Q the CRT's
-        generic-q reduction also for a near-power q = 2^W - gap."""
+        generic-q reduction also for a near-power q = 2^W - gap,
+        FLAG_MULMOD_TW6 the NTT kernels' six-operation twiddle products."""
         self._check(load_library().bcrt_set_kernel_flags(self._h, flags))
         self.kernel_flags = flags
 

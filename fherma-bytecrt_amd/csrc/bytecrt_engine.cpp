@@ -272,7 +272,7 @@ int create_engine(bcrt_engine** out, uint32_t max_batch, uint32_t N, uint32_t W,
         t.qshift = e->plan.qshift;
         t.enc_ts_log = std::min(5u, g.n1);
         t.fuse_min_tiles = bcrt::kFusedMinTiles;
-        t.kflags = 0;
+        t.kflags = e->plan.tw5_late ? bcrt::kPlanTw5Late : 0;
         t.gap = e->plan.gap;
         t.qt1 = e->plan.qt1;
         t.inv_qt1 = e->plan.inv_qt1;
@@ -528,9 +528,9 @@ extern "C" int bcrt_set_encode_tile(bcrt_engine* e, uint32_t ts_log) {
 extern "C" int bcrt_set_kernel_flags(bcrt_engine* e, uint32_t flags) {
     if (!e) return BCRT_EINVAL;
     return guarded(e, [&] {
-        if (flags > bcrt::kFlagsAll) throw std::invalid_argument("ByteCRT kernel flags must be within 0..63");
+        if (flags > bcrt::kFlagsAll) throw std::invalid_argument("ByteCRT kernel flags must be within 0..127");
         drop_graph(e);  // captured graphs hold the tables by value
-        e->tables.kflags = flags;
+        e->tables.kflags = flags | (e->tables.kflags & bcrt::kPlanFlags);
     });
 }
 

@@ -62,8 +62,18 @@ constexpr uint32_t kMaxPrimesCrt = 128;
 // ---------------------------------------------------------------------------
 constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
 
+// FP64 products are formed as fma(a, b, 0.0): on gfx950 v_mul_f64 costs more
+// issue time than v_fma_f64 (the same exact butterfly runs 17-23% faster with
+// the FMA in the butterfly probe).  fma(a, b, +0.0) is a * b rounded once,
+// the same double, except that a zero product is +0 where a * b may give -0;
+// every value here ends up as an integer (a conversion, a comparison with 0
+// or a sum with a non-zero integer), where the two zeros are the same.  The
+// compiler keeps the FMA (folding it into a multiply would change the sign of
+// zero) and encodes the 0.0 inline: no register.
+__device__ __forceinline__ double dmul(double a, double b) { return fma(a, b, 0.0); }
+
 __device__ __forceinline__ double mul_mod(double x, double w, double p, double pinv) {
-    const double h = x * w;
+    const double h = dmul(x, w);
     const double l = fma(x, w, -h);
     const double q = fma(h, pinv, kRound) - kRound;
     return fma(-q, p, h) + l;
@@ -81,25 +91,67 @@ __device__ __forceinline__ int32_t balanced(double x, double p, double pinv) {
 // u32 and balanced int32 words agree on [0, 2^31).
 __device__ __forceinline__ double word_value(uint32_t w) { return double(int32_t(w)); }
 
+// ---------------------------------------------------------------------------
+// Twiddle products in five FP64 operations (mul_mod takes six).  Every prime
+// of a plan has p = 1 (mod 2N), so p - 1 = k 2^s with s >= log2(2N) and
+// k < 2^(31 - s).  For w in [0, p), wq = fl(w * pinv) (one operation per
+// twiddle, shared by all its butterflies):
+//   q = rint(x wq): the FMA against 1.5 * 2^52 rounds the exact x wq once,
+//       and |wq - w/p| <= 2^-52, so |q - x w / p| <= 1/2 + |x| 2^-52;
+//   c = q p - q = q (p - 1) = (q k) 2^s by one FMA: exact while |q| k < 2^53;
+//   t = x w - c = (x w - q p) + q: one FMA, exact (an integer below 2^40);
+//   r = t - q = x w - q p, exact, with |r| <= p/2 + 2^-13 p for |x| < 2^38.
+// No TwoProduct is needed: the product's low part never has to be rounded.
+// Each caller states the largest |x| it passes and why |q| k < 2^53 holds
+// there (|q| < |x| + 1); mul_mod stays where inputs reach 2^48 (the fused
+// encode) or both factors vary (the pointwise product).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double mul_mod_tw(double x, double w, double wq, double p) {
+    const double q = fma(x, wq, kRound) - kRound;
+    return fma(x, w, -fma(q, p, -q)) - q;
+}
+
+// A twiddle's modular product: the five-operation form (FAST, wq given) or mul_mod.
+template <bool FAST>
+__device__ __forceinline__ double tw_mul(double x, double w, double wq, double p, double pinv) {
+    if constexpr (FAST)
+        return mul_mod_tw(x, w, wq, p);
+    else
+        return mul_mod(x, w, p, pinv);
+}
+
 // (x, y) -> (x + w y, x - w y)
-__device__ __forceinline__ void ct_bfly(double& x, double& y, double w, double p, double pinv) {
-    const double v = mul_mod(y, w, p, pinv);
+template <bool FAST = false>
+__device__ __forceinline__ void ct_bfly(double& x, double& y, double w, double wq, double p, double pinv) {
+    const double v = tw_mul<FAST>(y, w, wq, p, pinv);
     y = x - v;
     x = x + v;
 }
 
+__device__ __forceinline__ void ct_bfly(double& x, double& y, double w, double p, double pinv) {
+    ct_bfly<false>(x, y, w, 0.0, p, pinv);
+}
+
 // (x, y) -> (x + y, (x - y) w)
-__device__ __forceinline__ void gs_bfly(double& x, double& y, double w, double p, double pinv) {
+template <bool FAST = false>
+__device__ __forceinline__ void gs_bfly(double& x, double& y, double w, double wq, double p, double pinv) {
     const double d = x - y;
     x = x + y;
-    y = mul_mod(d, w, p, pinv);
+    y = tw_mul<FAST>(d, w, wq, p, pinv);
+}
+
+// The twiddle's quotient factor for the five-operation form (unused otherwise).
+template <bool FAST>
+__device__ __forceinline__ double tw_quot(double w, double pinv) {
+    return FAST ? dmul(w, pinv) : 0.0;
 }
 
 // K transform levels on the thread's 16 registers, which hold (16 >> K)
 // sub-blocks of 2^K consecutive registers.  Local level l is level e0 + l of
 // a length-M transform; the sub-block's outer index is o0 + sb; the twiddle
 // of group g is z[(mult << (e0 + l)) + ((o0 + sb) << l) + g].
-template <int K>
+// (FAST: five-operation twiddle products.)
+template <int K, bool FAST = false>
 __device__ __forceinline__ void ct_round(double (&v)[16], const uint32_t* __restrict__ z, uint32_t mult,
                                          uint32_t e0, uint32_t o0, double p, double pinv) {
 #pragma unroll
@@ -110,10 +162,11 @@ __device__ __forceinline__ void ct_round(double (&v)[16], const uint32_t* __rest
 #pragma unroll
             for (int g = 0; g < (1 << l); ++g) {
                 const double w = double(z[(mult << (e0 + l)) + ((o0 + sb) << l) + g]);
+                const double wq = tw_quot<FAST>(w, pinv);
 #pragma unroll
                 for (int x = 0; x < half; ++x) {
                     const int i0 = (sb << K) + g * 2 * half + x;
-                    ct_bfly(v[i0], v[i0 + half], w, p, pinv);
+                    ct_bfly<FAST>(v[i0], v[i0 + half], w, wq, p, pinv);
                 }
             }
         }
@@ -121,7 +174,7 @@ __device__ __forceinline__ void ct_round(double (&v)[16], const uint32_t* __rest
 }
 
 // (LMIN > 0 stops above the round's lowest LMIN levels.)
-template <int K, int LMIN = 0>
+template <int K, int LMIN = 0, bool FAST = false>
 __device__ __forceinline__ void gs_round(double (&v)[16], const uint32_t* __restrict__ z, uint32_t mult,
                                          uint32_t e0, uint32_t o0, double p, double pinv) {
 #pragma unroll
@@ -132,10 +185,11 @@ __device__ __forceinline__ void gs_round(double (&v)[16], const uint32_t* __rest
 #pragma unroll
             for (int g = 0; g < (1 << l); ++g) {
                 const double w = double(z[(mult << (e0 + l)) + ((o0 + sb) << l) + g]);
+                const double wq = tw_quot<FAST>(w, pinv);
 #pragma unroll
                 for (int x = 0; x < half; ++x) {
                     const int i0 = (sb << K) + g * 2 * half + x;
-                    gs_bfly(v[i0], v[i0 + half], w, p, pinv);
+                    gs_bfly<FAST>(v[i0], v[i0 + half], w, wq, p, pinv);
                 }
             }
         }
@@ -328,7 +382,7 @@ __device__ __forceinline__ void enc_compute(const DevTables& t, uint32_t* __rest
                 // qd = floor(u / p) or one less, so u - qd p (exact: both below 2^47)
                 // is in [0, 2p).
                 const double ud = fma(double(hi), 65536.0, double(lo)) + off[g];
-                const double qd = __builtin_trunc(ud * inv[g]);
+                const double qd = __builtin_trunc(dmul(ud, inv[g]));
                 const uint32_t r = uint32_t(fma(-qd, pd[g], ud));
                 const uint32_t j = pt * 8 + 2 * g + kh;
                 if (j < t.P) out[j * t.N + f] = min(r, r - uint32_t(pd[g]));
@@ -415,8 +469,19 @@ constexpr uint32_t exchange_stride() {
 // exchange between the two register rounds goes through LDS as FP64.
 // ---------------------------------------------------------------------------
 constexpr size_t kExchangeBytes = size_t(kThreads / 16) * exchange_stride<256>() * sizeof(double);
+// The column and middle kernels hold kExchangeBytes (35 KB) of LDS per
+// workgroup, so a CU runs four of them: four waves per SIMD.  Telling the
+// compiler so (instead of letting it aim for eight, which these kernels can
+// never reach) keeps it from trading latency for registers, such as issuing
+// each twiddle load just before its use.
+#define kNttOccupancy __attribute__((amdgpu_waves_per_eu(4, 4)))
+static_assert(4 * kExchangeBytes <= 160 * 1024 && 5 * kExchangeBytes > 160 * 1024, "four per CU");
 
-template <int NLOG1>
+// Forward columns take canonical encode words; level l adds at most p/2 +
+// 2^-13 p, so every twiddle product's input is below p + 7 (p/2 + 2^-13 p) <
+// 4.6 p < 2^33.3 and |q| k < 2^33.3 2^18 < 2^53 for s >= 13 (FAST holds for
+// every plan).
+template <int NLOG1, bool FAST = true>
 __device__ __forceinline__ void fwd_cols_wg(const DevTables& t, uint32_t* __restrict__ planes, uint32_t bx,
                                             uint32_t op, uint32_t prod, double* s) {
     constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = kThreads / T, S = exchange_stride<N1>();
@@ -432,7 +497,7 @@ __device__ __forceinline__ void fwd_cols_wg(const DevTables& t, uint32_t* __rest
     double v[16];
 #pragma unroll
     for (int x = 0; x < 16; ++x) v[x] = double(col[tau + T * x]);
-    ct_round<4>(v, z, 1, 0, 0, p, pinv);
+    ct_round<4, FAST>(v, z, 1, 0, 0, p, pinv);
     if constexpr (K2 > 0) {
         double* sc = s + cw * S;
 #pragma unroll
@@ -440,7 +505,7 @@ __device__ __forceinline__ void fwd_cols_wg(const DevTables& t, uint32_t* __rest
         wave_lds_sync();
 #pragma unroll
         for (int x = 0; x < 16; ++x) v[x] = sc[skew(16 * tau + x)];
-        ct_round<K2>(v, z, 1, 4, tau * (16 >> K2), p, pinv);
+        ct_round<K2, FAST>(v, z, 1, 4, tau * (16 >> K2), p, pinv);
 #pragma unroll
         for (int x = 0; x < 16; x += 4)
             *reinterpret_cast<uint4*>(col + 16 * tau + x) =
@@ -452,7 +517,12 @@ __device__ __forceinline__ void fwd_cols_wg(const DevTables& t, uint32_t* __rest
     }
 }
 
-template <int NLOG1>
+// Inverse columns take balanced words (|x| <= p/2 + 1 < m0 = p/2 + 2^-13 p);
+// each level at most doubles the largest value, so the inputs of the last
+// (folded) level are below 2^n1 m0, |q| < 2^(n1-1) p (1 + 2^-12) + 1, and
+// with s >= n1 + 9, |q| k < p^2 (1 + 2^-12) / 2^10 + 2^22 < 2^53 (FAST
+// holds for every plan).
+template <int NLOG1, bool FAST = true>
 __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __restrict__ planes, uint32_t bx,
                                             uint32_t prod, double* s) {
     constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = kThreads / T, S = exchange_stride<N1>();
@@ -474,7 +544,7 @@ __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __rest
             v[x + 2] = word_value(q4.z);
             v[x + 3] = word_value(q4.w);
         }
-        gs_round<K2>(v, z, 1, 4, tau * (16 >> K2), p, pinv);
+        gs_round<K2, 0, FAST>(v, z, 1, 4, tau * (16 >> K2), p, pinv);
         double* sc = s + cw * S;
 #pragma unroll
         for (int x = 0; x < 16; ++x) sc[skew(16 * tau + x)] = v[x];
@@ -485,62 +555,66 @@ __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __rest
 #pragma unroll
         for (int x = 0; x < 16; ++x) v[x] = word_value(col[x]);
     }
-    gs_round<4, 1>(v, z, 1, 0, 0, p, pinv);
+    gs_round<4, 1, FAST>(v, z, 1, 0, 0, p, pinv);
     // Level 0 (twiddle z[1]) with the scale folded in: (x + y) s and
     // (x - y) (z[1] s), one modular product per output instead of two for
     // the difference.
     const double sc2 = t.scale[j], sz = t.scale_z1[j];
+    const double sc2q = tw_quot<FAST>(sc2, pinv), szq = tw_quot<FAST>(sz, pinv);
 #pragma unroll
     for (int x = 0; x < 8; ++x) {
         const double d = v[x] - v[x + 8];
-        v[x] = mul_mod(v[x] + v[x + 8], sc2, p, pinv);
-        v[x + 8] = mul_mod(d, sz, p, pinv);
+        v[x] = tw_mul<FAST>(v[x] + v[x + 8], sc2, sc2q, p, pinv);
+        v[x + 8] = tw_mul<FAST>(d, sz, szq, p, pinv);
     }
 #pragma unroll
     for (int x = 0; x < 16; ++x) col[tau + T * x] = canon_product(v[x], p);
 }
 
 // Twiddles of a radix-16 round (K = 4, one sub-block): tw[2^l - 1 + g] is the
-// twiddle of group g at local level l, loaded ahead of the round that uses it.
-__device__ __forceinline__ void load_tw16(double (&tw)[15], const uint32_t* __restrict__ z, uint32_t mult,
+// twiddle of group g at local level l, loaded ahead of the round that uses it
+// and widened to FP64 there (u32 halves the registers they hold meanwhile).
+__device__ __forceinline__ void load_tw16(uint32_t (&tw)[15], const uint32_t* __restrict__ z, uint32_t mult,
                                           uint32_t e0, uint32_t o0) {
 #pragma unroll
     for (int l = 0; l < 4; ++l)
 #pragma unroll
-        for (int g = 0; g < (1 << l); ++g) tw[(1 << l) - 1 + g] = double(z[(mult << (e0 + l)) + (o0 << l) + g]);
+        for (int g = 0; g < (1 << l); ++g) tw[(1 << l) - 1 + g] = z[(mult << (e0 + l)) + (o0 << l) + g];
 }
 
 // Forward radix-16 round on two operands of the same prime and block at once:
 // each twiddle serves two independent butterfly chains.
-__device__ __forceinline__ void ct16x2_tw(double (&v)[16], double (&u)[16], const double (&tw)[15], double p,
+template <bool FAST>
+__device__ __forceinline__ void ct16x2_tw(double (&v)[16], double (&u)[16], const uint32_t (&tw)[15], double p,
                                           double pinv) {
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
         const int half = 1 << (3 - l);
 #pragma unroll
         for (int g = 0; g < (1 << l); ++g) {
-            const double w = tw[(1 << l) - 1 + g];
+            const double w = double(tw[(1 << l) - 1 + g]), wq = tw_quot<FAST>(w, pinv);
 #pragma unroll
             for (int x = 0; x < half; ++x) {
                 const int i0 = g * 2 * half + x;
-                ct_bfly(v[i0], v[i0 + half], w, p, pinv);
-                ct_bfly(u[i0], u[i0 + half], w, p, pinv);
+                ct_bfly<FAST>(v[i0], v[i0 + half], w, wq, p, pinv);
+                ct_bfly<FAST>(u[i0], u[i0 + half], w, wq, p, pinv);
             }
         }
     }
 }
 
-__device__ __forceinline__ void gs16_tw(double (&v)[16], const double (&tw)[15], double p, double pinv) {
+template <bool FAST>
+__device__ __forceinline__ void gs16_tw(double (&v)[16], const uint32_t (&tw)[15], double p, double pinv) {
 #pragma unroll
     for (int l = 3; l >= 0; --l) {
         const int half = 1 << (3 - l);
 #pragma unroll
         for (int g = 0; g < (1 << l); ++g) {
-            const double w = tw[(1 << l) - 1 + g];
+            const double w = double(tw[(1 << l) - 1 + g]), wq = tw_quot<FAST>(w, pinv);
 #pragma unroll
             for (int x = 0; x < half; ++x) {
                 const int i0 = g * 2 * half + x;
-                gs_bfly(v[i0], v[i0 + half], w, p, pinv);
+                gs_bfly<FAST>(v[i0], v[i0 + half], w, wq, p, pinv);
             }
         }
     }
@@ -553,6 +627,16 @@ __device__ __forceinline__ void gs16_tw(double (&v)[16], const double (&tw)[15],
 // twiddles stay in L2 across the whole batch; each radix-16 round's 15
 // twiddles are loaded while the round before it computes.  One exchange
 // buffer holds A and B (u32), then the product (FP64).
+//
+// Twiddle products (MODE): 0 mul_mod throughout; 1 the five-operation form
+// except in the last four inverse levels; 2 the five-operation form
+// throughout.  Bounds, with m0 = p/2 + 2^-13 p the largest modular product:
+// the forward rounds start from balanced words and add at most m0 per level
+// (inputs < 2.01 p); the pointwise product (mul_mod) returns at most m0;
+// inverse level i (i = 1 .. 8 after the product) takes inputs below 2^i m0.
+// So |q| k < 8.01 p 2^18 < 2^53 for i <= 4 and every plan (s >= 13), while
+// the last four levels reach |q| < 128 p + p / 32 + 1, which the plan checks
+// prime by prime (kPlanTw5Late; always true for N = 65,536, where s >= 17).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMidS = exchange_stride<256>();
 
@@ -564,9 +648,12 @@ __device__ __forceinline__ void mid_fetch(uint4 (&q4)[4], const uint32_t* __rest
     for (int k = 0; k < 4; ++k) q4[k] = src[k];
 }
 
+template <int MODE>
 __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restrict__ planes, uint32_t bx,
                                           uint32_t prod, uint32_t j, double* sx) {
     static_assert(kMidBlocks * kMidS * sizeof(double) <= kExchangeBytes, "exchange buffer");
+    static_assert(MODE >= 0 && MODE <= 2, "twiddle-product mode");
+    constexpr bool F1 = MODE >= 1, F2 = MODE == 2;
     const uint32_t b0 = bx * kMidBlocks;
     const double p = t.pd[2 * j], pinv = t.pd[2 * j + 1];
     uint32_t* pa = plane_ptr(t, planes, prod, 0, j);
@@ -583,7 +670,7 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
     uint32_t* rb = ub + bl * kMidS;
     double* rx = sx + bl * kMidS;
     double v[16], w[16];
-    double twa[15], twb[15];
+    uint32_t twa[15], twb[15];
     load_tw16(twa, zf, mult, 0, 0);
     {
         uint4 qa[4], qb[4];
@@ -610,7 +697,7 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
         w[x] = word_value(rb[skew(tau + 16 * x)]);
     }
     load_tw16(twb, zf, mult, 4, tau);
-    ct16x2_tw(v, w, twa, p, pinv);
+    ct16x2_tw<F1>(v, w, twa, p, pinv);
 #pragma unroll
     for (int x = 0; x < 16; ++x) {
         ra[skew(tau + 16 * x)] = uint32_t(balanced(v[x], p, pinv));
@@ -624,19 +711,19 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
         w[x] = word_value(rb[skew(16 * tau + x)]);
     }
     load_tw16(twa, zi, mult, 4, tau);
-    ct16x2_tw(v, w, twb, p, pinv);
+    ct16x2_tw<F1>(v, w, twb, p, pinv);
     // pointwise product (|v|, |w| < 2^34) and inverse sub-levels 7..4
 #pragma unroll
     for (int x = 0; x < 16; ++x) v[x] = mul_mod(v[x], w[x], p, pinv);
     load_tw16(twb, zi, mult, 0, 0);
-    gs16_tw(v, twa, p, pinv);
+    gs16_tw<F1>(v, twa, p, pinv);
     __syncthreads();
 #pragma unroll
     for (int x = 0; x < 16; ++x) rx[skew(16 * tau + x)] = v[x];
     wave_lds_sync();
 #pragma unroll
     for (int x = 0; x < 16; ++x) v[x] = rx[skew(tau + 16 * x)];
-    gs16_tw(v, twb, p, pinv);
+    gs16_tw<F2>(v, twb, p, pinv);
     __syncthreads();
     {
         // balanced residues back in place, then row u out as four 16-byte stores
@@ -1150,7 +1237,7 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
                     const uint32_t i = i0 + k;
                     if (i < L) {
                         const double tt = sv[k] + carry;
-                        carry = floor(tt * 0x1p-32);
+                        carry = floor(dmul(tt, 0x1p-32));
                         d[i * kCrtDigitStride] = uint32_t(fma(-carry, 4294967296.0, tt));
                     }
                 }
@@ -1169,7 +1256,7 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
                     if (i < L) {
                         const double qi = double(uint32_t(__builtin_amdgcn_readlane(int(qv), int(i))));
                         const double tt = fma(-Q, qi, sv[k]) + carry;
-                        carry = floor(tt * 0x1p-32);
+                        carry = floor(dmul(tt, 0x1p-32));
                         d[i * kCrtDigitStride] = uint32_t(fma(-carry, 4294967296.0, tt));
                     }
                 }
@@ -1338,27 +1425,28 @@ __global__ __launch_bounds__(kThreads, 3) void k_encode_fwd(DevTables t, const u
     encfwd_wg<KS, NLOG1>(t, a, b, planes, r, op, prod, smem_ef);
 }
 
-template <int NLOG1>
-__global__ __launch_bounds__(kThreads) void k_fwd_cols(DevTables t, uint32_t* __restrict__ planes) {
+template <int NLOG1, bool FAST>
+__global__ __launch_bounds__(kThreads) kNttOccupancy void k_fwd_cols(DevTables t, uint32_t* __restrict__ planes) {
     extern __shared__ __align__(16) uint8_t smem_fc[];
     uint32_t x, y, z;
     single_xyz(t, 1, &x, &y, &z);
-    fwd_cols_wg<NLOG1>(t, planes, x, y, z, reinterpret_cast<double*>(smem_fc));
+    fwd_cols_wg<NLOG1, FAST>(t, planes, x, y, z, reinterpret_cast<double*>(smem_fc));
 }
 
-template <int NLOG1>
-__global__ __launch_bounds__(kThreads) void k_inv_cols(DevTables t, uint32_t* __restrict__ planes) {
+template <int NLOG1, bool FAST>
+__global__ __launch_bounds__(kThreads) kNttOccupancy void k_inv_cols(DevTables t, uint32_t* __restrict__ planes) {
     extern __shared__ __align__(16) uint8_t smem_ic[];
     uint32_t x, y, z;
     single_xyz(t, 3, &x, &y, &z);
-    inv_cols_wg<NLOG1>(t, planes, x, z, reinterpret_cast<double*>(smem_ic));
+    inv_cols_wg<NLOG1, FAST>(t, planes, x, z, reinterpret_cast<double*>(smem_ic));
 }
 
-__global__ __launch_bounds__(kThreads) void k_middle(DevTables t, uint32_t* __restrict__ planes) {
+template <int MODE>
+__global__ __launch_bounds__(kThreads) kNttOccupancy void k_middle(DevTables t, uint32_t* __restrict__ planes) {
     extern __shared__ __align__(16) uint8_t smem_mid[];
     uint32_t x, y, z;
     single_xyz(t, 2, &x, &y, &z);
-    middle_wg(t, planes, x, y, z, reinterpret_cast<double*>(smem_mid));
+    middle_wg<MODE>(t, planes, x, y, z, reinterpret_cast<double*>(smem_mid));
 }
 
 // One kernel per k-step count, chosen on the host: each gets its own register
@@ -1459,7 +1547,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_encode_fwd_groups(const GroupTa
 }
 
 template <int NLOG1, bool PQ>
-__global__ __launch_bounds__(kThreads) void k_fwd_cols_groups(const GroupTable g) {
+__global__ __launch_bounds__(kThreads) kNttOccupancy void k_fwd_cols_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_fc_g[];
     for_each_tile<PQ>(g, 1, [](const GroupDesc& d, uint32_t x, uint32_t y, uint32_t z) {
         double* s = reinterpret_cast<double*>(smem_fc_g);
@@ -1477,16 +1565,19 @@ __global__ __launch_bounds__(kThreads) void k_fwd_cols_groups(const GroupTable g
     });
 }
 
-template <bool PQ>
-__global__ __launch_bounds__(kThreads) void k_middle_groups(const GroupTable g) {
+// (mixes use the five-operation twiddle products; MODE 2 when every group's
+// plan allows them in the last four inverse levels too)
+template <bool PQ, int MODE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(PQ ? 3 : 4, 4))) void k_middle_groups(
+    const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_mid_g[];
     for_each_tile<PQ>(g, 2, [](const GroupDesc& d, uint32_t x, uint32_t y, uint32_t z) {
-        middle_wg(d.t, d.args.planes, x, y, z, reinterpret_cast<double*>(smem_mid_g));
+        middle_wg<MODE>(d.t, d.args.planes, x, y, z, reinterpret_cast<double*>(smem_mid_g));
     });
 }
 
 template <int NLOG1, bool PQ>
-__global__ __launch_bounds__(kThreads) void k_inv_cols_groups(const GroupTable g) {
+__global__ __launch_bounds__(kThreads) kNttOccupancy void k_inv_cols_groups(const GroupTable g) {
     extern __shared__ __align__(16) uint8_t smem_ic_g[];
     for_each_tile<PQ>(g, 3, [](const GroupDesc& d, uint32_t x, uint32_t, uint32_t z) {
         double* s = reinterpret_cast<double*>(smem_ic_g);
@@ -1521,17 +1612,29 @@ bool shapes_ok(const DevTables& t, const BatchArgs& args) {
            t.Kcrt % 32 == 0 && t.Mcrt % 32 == 0 && t.P >= 1 && t.P <= kMaxPrimesCrt && t.L >= 3 &&
            t.L <= 64 && t.Kcrt >= 4 * (t.P + 1) && t.Mcrt >= 4 * t.L + 1 && t.Kenc >= 4 * t.L &&
            t.Kcrt <= 32 * 17 && t.Mcrt <= 32 * 4 * crt_tiles_per_wave(t.Kcrt / 32, t.Mcrt / 32) &&
-           t.qshift + 32 == t.W && t.enc_ts_log <= t.n1 && t.enc_ts_log <= 7 && t.kflags <= kFlagsAll;
+           t.qshift + 32 == t.W && t.enc_ts_log <= t.n1 && t.enc_ts_log <= 7 && (t.kflags & ~kPlanFlags) <= kFlagsAll;
+}
+
+// Twiddle-product form of the single-plan NTT kernels (see middle_wg): the
+// five-operation form unless kFlagMulModTw6 asks for mul_mod throughout; in
+// the middle kernel's last four inverse levels only where the plan allows it.
+int tw_mode(const DevTables& t) {
+    return (t.kflags & kFlagMulModTw6) ? 0 : (t.kflags & kPlanTw5Late) ? 2 : 1;
 }
 
 template <int NLOG1>
 hipError_t launch_cols(const DevTables& t, const BatchArgs& args, hipStream_t stream, bool forward) {
     constexpr uint32_t N1 = 1u << NLOG1, T = N1 / 16, CPW = kThreads / T;
     const dim3 grid(t.P * (t.N2 / CPW) * (forward ? 2 : 1) * args.batch);
-    if (forward)
-        hipLaunchKernelGGL(k_fwd_cols<NLOG1>, grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
+    const bool fast = tw_mode(t) > 0;
+    if (forward && fast)
+        hipLaunchKernelGGL((k_fwd_cols<NLOG1, true>), grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
+    else if (forward)
+        hipLaunchKernelGGL((k_fwd_cols<NLOG1, false>), grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
+    else if (fast)
+        hipLaunchKernelGGL((k_inv_cols<NLOG1, true>), grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
     else
-        hipLaunchKernelGGL(k_inv_cols<NLOG1>, grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
+        hipLaunchKernelGGL((k_inv_cols<NLOG1, false>), grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
     return hipGetLastError();
 }
 
@@ -1601,7 +1704,11 @@ hipError_t launch_forward_columns(const DevTables& t, const BatchArgs& args, hip
 hipError_t launch_middle(const DevTables& t, const BatchArgs& args, hipStream_t stream) {
     if (!shapes_ok(t, args)) return hipErrorInvalidValue;
     const dim3 grid(t.N1 / kMidBlocks * args.batch * t.P);
-    hipLaunchKernelGGL(k_middle, grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes);
+    switch (tw_mode(t)) {
+        case 0: hipLaunchKernelGGL(k_middle<0>, grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes); break;
+        case 1: hipLaunchKernelGGL(k_middle<1>, grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes); break;
+        default: hipLaunchKernelGGL(k_middle<2>, grid, dim3(kThreads), kExchangeBytes, stream, t, args.planes); break;
+    }
     return hipGetLastError();
 }
 
@@ -1754,7 +1861,12 @@ void enqueue_groups(const GroupDesc* groups, uint32_t ngroups, GroupTable& table
             default: go(reinterpret_cast<const void*>(&k_fwd_cols_groups<0, PQ>), total[1], kThreads, xb); break;
         }
     }
-    go(reinterpret_cast<const void*>(&k_middle_groups<PQ>), total[2], kThreads, xb);
+    bool late = true;
+    for (uint32_t k = 0; k < ngroups; ++k) late = late && (groups[k].t.kflags & kPlanTw5Late);
+    if (late)
+        go(reinterpret_cast<const void*>(&k_middle_groups<PQ, 2>), total[2], kThreads, xb);
+    else
+        go(reinterpret_cast<const void*>(&k_middle_groups<PQ, 1>), total[2], kThreads, xb);
     switch (n1) {
         case 4: go(reinterpret_cast<const void*>(&k_inv_cols_groups<4, PQ>), total[3], kThreads, xb); break;
         case 5: go(reinterpret_cast<const void*>(&k_inv_cols_groups<5, PQ>), total[3], kThreads, xb); break;

@@ -38,7 +38,13 @@ constexpr uint32_t kFlagWeightsL2 = 4;  // fused encode: weight fragments from L
 constexpr uint32_t kFlagCrtWide = 8;    // CRT built for five workgroups per CU, weights one k-step ahead
 constexpr uint32_t kFlagEncOneChain = 16;  // separate encode: one MFMA chain per wave, four waves per SIMD
 constexpr uint32_t kFlagGenericQ = 32;     // CRT: the generic-q reduction also for q = 2^W - gap
-constexpr uint32_t kFlagsAll = 63;
+constexpr uint32_t kFlagMulModTw6 = 64;    // NTT kernels: six-operation twiddle products (mul_mod) throughout
+constexpr uint32_t kFlagsAll = 127;
+// Set by the engine from its plan, not a choice: every prime allows the
+// five-operation twiddle products in the middle kernel's last four inverse
+// levels (|q| k < 2^53 for |q| < 128 p + p / 32 + 1, where p - 1 = k 2^s).
+constexpr uint32_t kPlanTw5Late = 1u << 31;
+constexpr uint32_t kPlanFlags = kPlanTw5Late;
 
 // One batch of products.  a, b, out: [batch][N][L] u32 coefficient-major.
 // planes: [batch][2][P][N] u32 workspace (operand A planes end up holding the

@@ -258,8 +258,18 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
     pl.wenc.assign(size_t(g.PTenc) * 32 * g.Kenc, 0);
 
     std::vector<uint32_t> pw(N), pwi(N);
+    pl.tw5_late = true;
     for (uint32_t j = 0; j < P; ++j) {
         const uint32_t p = pl.primes[j];
+        // The middle kernel's last four inverse levels pass inputs up to
+        // 128 p + p / 32 to twiddle products whose quotient q times the odd
+        // part k of p - 1 must stay below 2^53 (exact q (p - 1)).
+        {
+            uint32_t k = p - 1;
+            while (!(k & 1)) k >>= 1;
+            const unsigned __int128 qmax = uint64_t(128) * p + p / 32 + 2;
+            if (qmax * k >= (unsigned __int128)1 << 53) pl.tw5_late = false;
+        }
         // Smallest base a >= 2 with (a^((p-1)/2N))^N == -1 (rns-plan.hpp:163-170).
         uint32_t psi = 0;
         for (uint32_t a = 2; a < p && !psi; ++a) {

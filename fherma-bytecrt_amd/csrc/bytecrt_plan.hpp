@@ -61,6 +61,7 @@ struct HostPlan {
     uint64_t qt1 = 0;                   // (q >> qshift) + 1
     double inv_qt1 = 0;                 // 1 / qt1
     uint32_t gap = 0;                   // 2^W - q when below 2^32 and L >= 3 (near-power CRT reduction), else 0
+    bool tw5_late = false;              // every prime: |q| k < 2^53 for |q| < 128 p + p / 32 + 1 (p - 1 = k 2^s)
 };
 
 // Throws std::invalid_argument (bad geometry or q) or std::runtime_error (no plan).

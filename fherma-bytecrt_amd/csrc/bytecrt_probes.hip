@@ -70,6 +70,24 @@ __device__ __forceinline__ double mul_mod_f64(double x, double w, double p, doub
     return fma(-q, p, h) + l;
 }
 
+// The same with the product x w formed by an FMA with a -0.0 addend (the
+// same double, bit for bit): v_fma_f64 instead of v_mul_f64.  nz must be
+// -0.0 held where the compiler cannot see it (it would fold the FMA back).
+__device__ __forceinline__ double mul_mod_f64_fma(double x, double w, double p, double pinv, double nz) {
+    const double h = fma(x, w, nz);
+    const double l = fma(x, w, -h);
+    const double q = fma(h, pinv, kRound) - kRound;
+    return fma(-q, p, h) + l;
+}
+
+// The NTT kernels' five-operation FP64 form for p - 1 = k 2^s: wq = fl(w/p),
+// q = rint(x wq), x w - q p = fma(x, w, -(q p - q)) - q, exact while |q| k < 2^53
+// (the probe's inputs stay below 2.1 p; the entry point checks 3 p k < 2^53).
+__device__ __forceinline__ double mul_mod_tw(double x, double w, double wq, double p) {
+    const double q = fma(x, wq, kRound) - kRound;
+    return fma(x, w, -fma(q, p, -q)) - q;
+}
+
 // x in [0, p), w in [0, p), ws = floor(w 2^32 / p): x w mod p in [0, p).
 __device__ __forceinline__ uint32_t mul_mod_shoup(uint32_t x, uint32_t w, uint32_t ws, uint32_t p) {
     const uint32_t q = __umulhi(x, ws);
@@ -108,7 +126,76 @@ template <int FORM>
 __global__ __launch_bounds__(kThreads) void k_bfly(uint32_t* __restrict__ out, const Twiddles tw, uint32_t rounds) {
     const size_t tid = size_t(blockIdx.x) * kThreads + threadIdx.x;
     const uint32_t p = tw.p;
-    if constexpr (FORM == 0) {
+    if constexpr (FORM == 3 || FORM == 4) {
+        // five-operation form; FORM 4 writes the two butterflies of each pair
+        // (consecutive in the level's order) step by step, interleaved
+        double v[16], w[15], wq[15];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {
+            w[k] = double(tw.w[k]);
+            wq[k] = w[k] * tw.pinv;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = double(start_value(tid, k, p));
+        for (uint32_t r = 0; r < rounds; ++r) {
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const int half = 8 >> l;
+#pragma unroll
+                for (int b = 0; b < 8; b += 2) {
+                    int i0[2], k[2];
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const int g = (b + u) / half, x = (b + u) % half;
+                        i0[u] = g * 2 * half + x;
+                        k[u] = (1 << l) - 1 + g;
+                    }
+                    if constexpr (FORM == 3) {
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            const double t = mul_mod_tw(v[i0[u] + half], w[k[u]], wq[k[u]], tw.pd);
+                            v[i0[u] + half] = v[i0[u]] - t;
+                            v[i0[u]] = v[i0[u]] + t;
+                        }
+                    } else {
+                        // (sched_barrier keeps the compiler from putting each
+                        // chain back together: it treats the FP64 results as
+                        // available at once)
+                        double q[2], c[2], t[2];
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) q[u] = fma(v[i0[u] + half], wq[k[u]], kRound);
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) q[u] = q[u] - kRound;
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) c[u] = fma(q[u], tw.pd, -q[u]);
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) t[u] = fma(v[i0[u] + half], w[k[u]], -c[u]);
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) t[u] = t[u] - q[u];
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            v[i0[u] + half] = v[i0[u]] - t[u];
+                            v[i0[u]] = v[i0[u]] + t[u];
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = fma(-(fma(v[k], tw.pinv, kRound) - kRound), tw.pd, v[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const double r = v[k] < 0.0 ? v[k] + tw.pd : v[k];
+            out[tid * 16 + k] = uint32_t(r);
+        }
+    } else if constexpr (FORM == 0 || FORM == 5) {
+        double nz = -0.0;
+        __asm__ volatile("" : "+v"(nz));  // opaque: keeps fma(x, w, nz) an FMA
         double v[16], w[15];
 #pragma unroll
         for (int k = 0; k < 15; ++k) w[k] = double(tw.w[k]);
@@ -123,7 +210,9 @@ __global__ __launch_bounds__(kThreads) void k_bfly(uint32_t* __restrict__ out, c
 #pragma unroll
                     for (int x = 0; x < half; ++x) {
                         const int i0 = g * 2 * half + x;
-                        const double t = mul_mod_f64(v[i0 + half], w[(1 << l) - 1 + g], tw.pd, tw.pinv);
+                        const double t =
+                            FORM == 0 ? mul_mod_f64(v[i0 + half], w[(1 << l) - 1 + g], tw.pd, tw.pinv)
+                                      : mul_mod_f64_fma(v[i0 + half], w[(1 << l) - 1 + g], tw.pd, tw.pinv, nz);
                         v[i0 + half] = v[i0] - t;
                         v[i0] = v[i0] + t;
                     }
@@ -280,10 +369,15 @@ extern "C" int bcrtp_hbm_stream(int mode, const void* d_src, void* d_dst, size_t
 }
 
 extern "C" int bcrtp_butterflies(int form, uint32_t* d_out, size_t threads, uint32_t p, const uint32_t* tw15,
-                                 uint32_t rounds, void* stream) {
-    if (form < 0 || form > 2 || !d_out || !tw15 || threads == 0 || threads % kThreads || p <= (1u << 30) ||
-        p >= (1u << 31) || (p & 1) == 0 || threads / kThreads > 0x7FFFFFFF)
+                                 uint32_t rounds, uint32_t lds_bytes, void* stream) {
+    if (form < 0 || form > 5 || !d_out || !tw15 || threads == 0 || threads % kThreads || p <= (1u << 30) ||
+        p >= (1u << 31) || (p & 1) == 0 || threads / kThreads > 0x7FFFFFFF || lds_bytes > 65536)
         return BCRT_EINVAL;
+    if (form == 3 || form == 4) {
+        uint32_t k = p - 1;
+        while (!(k & 1)) k >>= 1;
+        if ((unsigned __int128)(uint64_t(3) * p) * k >= (unsigned __int128)1 << 53) return BCRT_EINVAL;
+    }
     Twiddles tw{};
     tw.p = p;
     uint32_t inv = 1;  // p^-1 mod 2^32 by Newton iteration (p odd)
@@ -299,8 +393,11 @@ extern "C" int bcrtp_butterflies(int form, uint32_t* d_out, size_t threads, uint
     }
     const dim3 grid(uint32_t(threads / kThreads)), blk(kThreads);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (form == 0) hipLaunchKernelGGL(k_bfly<0>, grid, blk, 0, s, d_out, tw, rounds);
-    if (form == 1) hipLaunchKernelGGL(k_bfly<1>, grid, blk, 0, s, d_out, tw, rounds);
-    if (form == 2) hipLaunchKernelGGL(k_bfly<2>, grid, blk, 0, s, d_out, tw, rounds);
+    if (form == 0) hipLaunchKernelGGL(k_bfly<0>, grid, blk, lds_bytes, s, d_out, tw, rounds);
+    if (form == 1) hipLaunchKernelGGL(k_bfly<1>, grid, blk, lds_bytes, s, d_out, tw, rounds);
+    if (form == 2) hipLaunchKernelGGL(k_bfly<2>, grid, blk, lds_bytes, s, d_out, tw, rounds);
+    if (form == 3) hipLaunchKernelGGL(k_bfly<3>, grid, blk, lds_bytes, s, d_out, tw, rounds);
+    if (form == 4) hipLaunchKernelGGL(k_bfly<4>, grid, blk, lds_bytes, s, d_out, tw, rounds);
+    if (form == 5) hipLaunchKernelGGL(k_bfly<5>, grid, blk, lds_bytes, s, d_out, tw, rounds);
     return hipGetLastError() == hipSuccess ? BCRT_OK : BCRT_ERUNTIME;
 }

@@ -17,7 +17,10 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "lib" / "libbytecrt_probes.so"
 EXPORTED = ("bcrtp_hbm_stream", "bcrtp_butterflies", "bcrtp_issue_overlap", "bcrtp_issue_overlap_words")
 KINDS = {0: "fp64_fma", 1: "fp32_fma", 2: "int24_mad"}
-FORMS = {0: "fp64", 1: "shoup32", 2: "montgomery32"}
+FORMS = {0: "fp64", 1: "shoup32", 2: "montgomery32", 3: "fp64_tw5", 4: "fp64_tw5_pairs", 5: "fp64_fma_product"}
+# Dynamic LDS per 256-thread workgroup that holds a CU to four of them: four
+# waves per SIMD, the occupancy of the middle NTT kernel.
+LDS_FOUR_WAVES = 40 * 1024
 MODES = {0: "copy", 1: "read", 2: "write"}
 _LIB = None
 
@@ -33,7 +36,7 @@ def load_library():
         lib.bcrtp_hbm_stream.argtypes = [ctypes.c_int, vp, vp, ctypes.c_size_t, vp]
         lib.bcrtp_butterflies.restype = ctypes.c_int
         lib.bcrtp_butterflies.argtypes = [ctypes.c_int, vp, ctypes.c_size_t, ctypes.c_uint32,
-                                          ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, vp]
+                                          ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.c_uint32, vp]
         lib.bcrtp_issue_overlap.restype = ctypes.c_int
         lib.bcrtp_issue_overlap.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, vp,
                                             ctypes.c_size_t, vp]
@@ -50,9 +53,9 @@ def hbm_stream(mode: int, src, dst, nbytes: int, stream=None):
         raise RuntimeError(f"ByteCRT HBM probe mode {mode} failed ({rc})")
 
 
-def butterflies(form: int, out, threads: int, p: int, tw15, rounds: int, stream=None):
+def butterflies(form: int, out, threads: int, p: int, tw15, rounds: int, stream=None, lds_bytes: int = 0):
     arr = (ctypes.c_uint32 * 15)(*[int(w) for w in tw15])
-    rc = load_library().bcrtp_butterflies(form, out.data_ptr(), threads, p, arr, rounds,
+    rc = load_library().bcrtp_butterflies(form, out.data_ptr(), threads, p, arr, rounds, lds_bytes,
                                          stream.cuda_stream if stream is not None else None)
     if rc != 0:
         raise RuntimeError(f"ByteCRT butterfly probe form {form} failed ({rc})")
@@ -99,18 +102,26 @@ def measure_hbm(torch, nbytes: int = 2 << 30, reps: int = 5) -> dict:
 
 
 def measure_butterflies(torch, p: int, tw15, threads: int = 1 << 20, rounds: int = 64, reps: int = 5) -> dict:
-    """Butterflies per second (x 1e12) of each form, and whether all forms
-    produced the same residues."""
+    """Butterflies per second (x 1e12) of each form, with as many waves per
+    SIMD as registers allow and (``*_4w``) with four, and whether all forms
+    produced the same residues.  The five-operation forms need p - 1 = k 2^s
+    with 3 p k < 2^53 (true for the primes of N = 65,536); for other primes
+    they are left out."""
     stream = torch.cuda.Stream()
     outs, res = {}, {"threads": threads, "rounds": rounds, "p": p}
+    k = p - 1
+    while k % 2 == 0:
+        k //= 2
     for form, name in FORMS.items():
-        out = torch.empty(threads * 16, dtype=torch.int32, device="cuda")
-        ms = _timed(torch, stream, lambda: butterflies(form, out, threads, p, tw15, rounds, stream), reps)
-        res[f"{name}_Tbfly_per_s"] = round(threads * rounds * 32 / (ms * 1e-3) / 1e12, 3)
-        outs[name] = out
+        if form in (3, 4) and 3 * p * k >= 1 << 53:
+            continue
+        for lds, suffix in ((0, ""), (LDS_FOUR_WAVES, "_4w")):
+            out = torch.empty(threads * 16, dtype=torch.int32, device="cuda")
+            ms = _timed(torch, stream, lambda: butterflies(form, out, threads, p, tw15, rounds, stream, lds), reps)
+            res[f"{name}{suffix}_Tbfly_per_s"] = round(threads * rounds * 32 / (ms * 1e-3) / 1e12, 3)
+            outs[name + suffix] = out
     torch.cuda.synchronize()
-    res["forms_agree"] = bool(torch.equal(outs["fp64"], outs["shoup32"]) and
-                              torch.equal(outs["fp64"], outs["montgomery32"]))
+    res["forms_agree"] = all(torch.equal(outs["fp64"], o) for o in outs.values())
     return res
 
 

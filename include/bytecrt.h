@@ -127,9 +127,11 @@ int bcrt_set_fusion(bcrt_engine* e, uint32_t min_tiles);
  * workgroups per CU (instead of two chains at three); 32 makes the CRT reduce
  * a near-power modulus q = 2^W - gap (0 < gap < 2^32) by the generic-q steps
  * instead of its near-power steps (g1-crt.cuh:15-68; the tuner leaves this bit
- * to tests, which compare the two).  Default 0.  Results do not depend on
- * them.  Applies to later runs and later mixes (bits 8 and 16 act on
- * single-plan launches only). */
+ * to tests, which compare the two); 64 makes the NTT kernels multiply by
+ * twiddles with the six-operation TwoProduct form instead of the
+ * five-operation form exact for primes p = 1 mod 2N (tests compare the two).
+ * Default 0.  Results do not depend on them.  Applies to later runs and later
+ * mixes (bits 8, 16 and 64 act on single-plan launches only). */
 int bcrt_set_kernel_flags(bcrt_engine* e, uint32_t flags);
 
 /* ---- Mixed batches (products of different N, W, q, one launch per stage) ----

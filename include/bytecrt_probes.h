@@ -33,9 +33,18 @@ int bcrtp_hbm_stream(int mode, const void* d_src, void* d_dst, size_t bytes, voi
  * form 0: FP64 (TwoProduct, rounded quotient, exact remainder; the product
  *         kernels' arithmetic; values balanced again after every round);
  * form 1: 32-bit Shoup (precomputed floor(w 2^32 / p), v_mul_hi_u32);
- * form 2: 32-bit Montgomery (R = 2^32).
- * 2^30 < p < 2^31 prime.  Every form computes the same residues. */
+ * form 2: 32-bit Montgomery (R = 2^32);
+ * form 3: FP64 in five operations for p - 1 = k 2^s (q = rint(x w/p),
+ *         fma(x, w, -(q p - q)) - q; needs 3 p k < 2^53, else BCRT_EINVAL);
+ * form 4: form 3 with each pair of butterflies written step by step,
+ *         interleaved;
+ * form 5: form 0 with the product x w formed by an FMA with a -0.0 addend
+ *         (v_fma_f64 instead of v_mul_f64; the same doubles).
+ * 2^30 < p < 2^31 prime.  Every form computes the same residues.
+ * lds_bytes (<= 65536) of dynamic LDS per 256-thread workgroup, which only
+ * limits how many waves share a SIMD (0: registers decide). */
 int bcrtp_butterflies(int form, uint32_t* d_out, size_t threads, uint32_t p, const uint32_t* tw15, uint32_t rounds,
+                      uint32_t lds_bytes,
                       void* stream);
 
 /* Overlap of matrix-core and vector work on one SIMD: a grid of two 512-thread

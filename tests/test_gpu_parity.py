@@ -50,12 +50,18 @@ def test_encode_stage(pkg, orc, gu, N, W):
         np.testing.assert_array_equal(got[op], want, err_msg=f"operand {op}")
 
 
-@pytest.mark.parametrize("N,W", [(4096, 100), (32768, 868), (65536, 1747)])
-def test_ntt_stages_dense_protocol(pkg, orc, gu, N, W):
+@pytest.mark.parametrize("tw6", [False, True], ids=["tw5", "tw6"])
+@pytest.mark.parametrize("N,W", [(4096, 100), (16384, 436), (32768, 868), (65536, 1747)])
+def test_ntt_stages_dense_protocol(pkg, orc, gu, N, W, tw6):
     """ntt-smoke.cu protocol: dense splitmix inputs with boundary words per prime;
-    the three NTT stages leave t_j = (a*b mod (X^N+1, p_j)) (P/p_j)^-1 mod p_j."""
+    the three NTT stages leave t_j = (a*b mod (X^N+1, p_j)) (P/p_j)^-1 mod p_j.
+    Both twiddle-product forms: the five-operation one (the default; at N <=
+    16,384 the middle kernel's last four inverse levels keep mul_mod, its
+    plan check fails there) and mul_mod throughout (FLAG_MULMOD_TW6)."""
     q = pkg.datagen.modulus(W, 3)
     e = pkg.Engine(N, W, q)
+    if tw6:
+        e.set_kernel_flags(e.FLAG_MULMOD_TW6)
     pl = orc.Plan(N, W, q)
     nat = np.zeros((2, e.P, N), dtype=np.uint32)
     want = np.zeros((e.P, N), dtype=np.uint32)
@@ -165,6 +171,35 @@ def test_batch_off_diagonal_geometries(pkg, orc, gu, N, W):
     for i in range(B):
         want = plan.product(a[i], b[i])
         bad = np.argwhere((got[i] != want).any(axis=1))
+        assert bad.size == 0, f"product {i}: {len(bad)} mismatches, first {bad[:4].ravel().tolist()}"
+
+
+@pytest.mark.parametrize("N,W", [(8192, 220), (32768, 868), (65536, 1747)])
+def test_twiddle_forms_agree(pkg, orc, gu, N, W):
+    """The NTT kernels' five-operation twiddle products (the default) and the
+    six-operation TwoProduct form (FLAG_MULMOD_TW6) give the same products
+    (a batch of two with 10% edge coefficients and all-(q-1) rows planted),
+    both equal to the oracle."""
+    B = 2
+    q = pkg.datagen.modulus(W, 41)
+    e = pkg.Engine(N, W, q, max_batch=B)
+    a = np.stack([pkg.datagen.operands(N, q, 1600 + i, edges_first=(i == 0), edge_fraction=0.1) for i in range(B)])
+    b = np.stack([pkg.datagen.operands(N, q, 1700 + i, edge_fraction=0.1) for i in range(B)])
+    qm1 = orc.int_to_limbs(q - 1, e.L)
+    a[1, :64] = qm1
+    b[1, :64] = qm1
+    da, db = gu.to_dev(a), gu.to_dev(b)
+    got = {}
+    for flags in (0, e.FLAG_MULMOD_TW6):
+        e.set_kernel_flags(flags)
+        out = gu.empty_dev(B * N * e.L)
+        e.run_device(da, db, out, B)
+        got[flags] = gu.from_dev(out).reshape(B, N, e.L)
+    np.testing.assert_array_equal(got[0], got[e.FLAG_MULMOD_TW6])
+    plan = orc.Plan(N, W, q)
+    for i in range(B):
+        want = plan.product(a[i], b[i])
+        bad = np.argwhere((got[0][i] != want).any(axis=1))
         assert bad.size == 0, f"product {i}: {len(bad)} mismatches, first {bad[:4].ravel().tolist()}"
 
 

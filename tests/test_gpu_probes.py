@@ -40,9 +40,11 @@ def test_butterfly_forms_agree_with_model(pkg):
         tw = [pow(psi, k, p) for k in range(1, 16)]
         threads, rounds = 512, 3
         outs = []
-        for form in (0, 1, 2):
+        # the five-operation forms (3, 4; primes of N = 65,536 have s >= 17)
+        # also at four waves per SIMD
+        for form, lds in ((0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (4, pkg.probes.LDS_FOUR_WAVES), (5, 0)):
             out = torch.empty(threads * 16, dtype=torch.int32, device="cuda")
-            pkg.probes.butterflies(form, out, threads, p, tw, rounds)
+            pkg.probes.butterflies(form, out, threads, p, tw, rounds, lds_bytes=lds)
             torch.cuda.synchronize()
             outs.append(out.cpu().numpy().view("uint32"))
         for o in outs[1:]:
