@@ -626,15 +626,18 @@ __device__ __forceinline__ void gs16_tw(double (&v)[16], const uint32_t (&tw)[15
 // Grid (N1/16, batch, P): workgroups of one prime run together so its block
 // twiddles stay in L2 across the whole batch; each radix-16 round's 15
 // twiddles are loaded while the round before it computes.  One exchange
-// buffer holds A and B (u32), then the product (FP64).
+// buffer holds the tile of A and B (u32), then each wave's rows of A, of B
+// and of the product in turn (FP64).
 //
 // Twiddle products (MODE): 0 mul_mod throughout; 1 the five-operation form
 // except in the last four inverse levels; 2 the five-operation form
 // throughout.  Bounds, with m0 = p/2 + 2^-13 p the largest modular product:
-// the forward rounds start from balanced words and add at most m0 per level
-// (inputs < 2.01 p); the pointwise product (mul_mod) returns at most m0;
+// the forward levels start from balanced words and add at most m0 per level
+// (inputs < p/2 + 1 + 7 m0 < 4.01 p); the pointwise product (mul_mod)
+// returns at most m0;
 // inverse level i (i = 1 .. 8 after the product) takes inputs below 2^i m0.
-// So |q| k < 8.01 p 2^18 < 2^53 for i <= 4 and every plan (s >= 13), while
+// So |q| k < 8.01 p 2^18 < 2^53 for the forward levels and for i <= 4 and
+// every plan (s >= 13), while
 // the last four levels reach |q| < 128 p + p / 32 + 1, which the plan checks
 // prime by prime (kPlanTw5Late; always true for N = 65,536, where s >= 17).
 // ---------------------------------------------------------------------------
@@ -662,12 +665,16 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
     const uint32_t mult = t.N1 + b0 + bl;
     const uint32_t* zf = t.zf + size_t(j) * t.N;
     const uint32_t* zi = t.zi + size_t(j) * t.N;
-    // The exchange buffer holds both operands as 32-bit residue words (A in the first
-    // half, B in the second) for the forward sub-levels, then the product as FP64.
-    uint32_t* ua = reinterpret_cast<uint32_t*>(sx);
-    uint32_t* ub = ua + kMidBlocks * kMidS;
-    uint32_t* ra = ua + bl * kMidS;
-    uint32_t* rb = ub + bl * kMidS;
+    // The exchange buffer first holds both operands as 32-bit residue words,
+    // laid out per wave: wave w's four blocks of A, then its four of B, in
+    // bytes [32 w kMidS, 32 (w + 1) kMidS), which is exactly where its blocks'
+    // FP64 rows (kMidS doubles per block) lie.  So every later exchange of a
+    // wave (A, then B, then the product, each as FP64) touches only its own
+    // bytes, and needs only a wave-level sync.
+    uint32_t* uw = reinterpret_cast<uint32_t*>(sx);
+    auto arow = [](uint32_t blk) { return (blk >> 2) * (8 * kMidS) + (blk & 3) * kMidS; };
+    uint32_t* ra = uw + arow(bl);
+    uint32_t* rb = ra + 4 * kMidS;
     double* rx = sx + bl * kMidS;
     double v[16], w[16];
     uint32_t twa[15], twb[15];
@@ -679,14 +686,15 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
         const uint32_t o = skew(threadIdx.x);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            ua[(4 * k + 0) * kMidS + o] = qa[k].x;
-            ua[(4 * k + 1) * kMidS + o] = qa[k].y;
-            ua[(4 * k + 2) * kMidS + o] = qa[k].z;
-            ua[(4 * k + 3) * kMidS + o] = qa[k].w;
-            ub[(4 * k + 0) * kMidS + o] = qb[k].x;
-            ub[(4 * k + 1) * kMidS + o] = qb[k].y;
-            ub[(4 * k + 2) * kMidS + o] = qb[k].z;
-            ub[(4 * k + 3) * kMidS + o] = qb[k].w;
+            uint32_t* da = uw + k * (8 * kMidS) + o;  // blocks 4k .. 4k + 3 (wave k's), A then B
+            da[0 * kMidS] = qa[k].x;
+            da[1 * kMidS] = qa[k].y;
+            da[2 * kMidS] = qa[k].z;
+            da[3 * kMidS] = qa[k].w;
+            da[4 * kMidS] = qb[k].x;
+            da[5 * kMidS] = qb[k].y;
+            da[6 * kMidS] = qb[k].z;
+            da[7 * kMidS] = qb[k].w;
         }
     }
     __syncthreads();
@@ -698,26 +706,29 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
     }
     load_tw16(twb, zf, mult, 4, tau);
     ct16x2_tw<F1>(v, w, twa, p, pinv);
+    // Exchange A, then B, as FP64 through the block's row: no reduction (the
+    // values stay below p/2 + 1 + 4 m0 < 2.51 p) and no conversions.
+    wave_lds_sync();  // this wave's u32 rows (under its FP64 rows) are read
 #pragma unroll
-    for (int x = 0; x < 16; ++x) {
-        ra[skew(tau + 16 * x)] = uint32_t(balanced(v[x], p, pinv));
-        rb[skew(tau + 16 * x)] = uint32_t(balanced(w[x], p, pinv));
-    }
+    for (int x = 0; x < 16; ++x) rx[skew(tau + 16 * x)] = v[x];
+    wave_lds_sync();
+#pragma unroll
+    for (int x = 0; x < 16; ++x) v[x] = rx[skew(16 * tau + x)];
+    wave_lds_sync();
+#pragma unroll
+    for (int x = 0; x < 16; ++x) rx[skew(tau + 16 * x)] = w[x];
     wave_lds_sync();
     // sub-levels 4..7 (positions 16 tau + x)
 #pragma unroll
-    for (int x = 0; x < 16; ++x) {
-        v[x] = word_value(ra[skew(16 * tau + x)]);
-        w[x] = word_value(rb[skew(16 * tau + x)]);
-    }
+    for (int x = 0; x < 16; ++x) w[x] = rx[skew(16 * tau + x)];
     load_tw16(twa, zi, mult, 4, tau);
     ct16x2_tw<F1>(v, w, twb, p, pinv);
-    // pointwise product (|v|, |w| < 2^34) and inverse sub-levels 7..4
+    // pointwise product (|v|, |w| < 2.51 p + 4 m0 < 2^33.2) and inverse sub-levels 7..4
 #pragma unroll
     for (int x = 0; x < 16; ++x) v[x] = mul_mod(v[x], w[x], p, pinv);
     load_tw16(twb, zi, mult, 0, 0);
     gs16_tw<F1>(v, twa, p, pinv);
-    __syncthreads();
+    wave_lds_sync();  // every lane of the wave has read B from the row
 #pragma unroll
     for (int x = 0; x < 16; ++x) rx[skew(16 * tau + x)] = v[x];
     wave_lds_sync();
