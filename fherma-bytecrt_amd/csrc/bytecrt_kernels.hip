@@ -1140,7 +1140,7 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const uint32_t j = 4 * (g0 + 4 * it) + k;
-                if (j < P) part += double(v[it][k]) * t.invp[j];
+                if (j < P) part = fma(double(v[it][k]), t.invp[j], part);  // one rounding per term
             }
         }
         // A share (at most 36 terms, each t_j / p_j < 1) is below 2^6, so FP32
