@@ -620,13 +620,13 @@ def main():
     # NTT butterflies per batch (2 forward + 1 inverse transform per prime,
     # N/2 log2 N each) at the measured rates of the forms the kernels use: the
     # fused encode's forward column levels (n1 per operand) the six-operation
-    # form with FMA products, the middle and inverse-column levels the
-    # five-operation form (both at four waves per SIMD, the kernels' occupancy)
+    # form, the middle and inverse-column levels the five-operation form
+    # (three and four waves per SIMD, the kernels' occupancy)
     bf = ceilings["butterflies"]
     n1 = (N // 256).bit_length() - 1
     cols_bfly = 2 * P * (N // 2) * n1 * B
     rest_bfly = 3 * P * (N // 2) * 16 * B - cols_bfly
-    ceilings["ntt_fp64_floor_ms"] = round((cols_bfly / bf["fp64_fma_product_4w_Tbfly_per_s"] +
+    ceilings["ntt_fp64_floor_ms"] = round((cols_bfly / bf["fp64_4w_Tbfly_per_s"] +
                                            rest_bfly / bf["fp64_tw5_4w_Tbfly_per_s"]) / 1e12 * 1e3, 3)
     # NTT-only kernels of the product path (the forward column levels run
     # inside the fused encode kernel)

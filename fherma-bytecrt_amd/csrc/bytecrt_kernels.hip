@@ -62,18 +62,8 @@ constexpr uint32_t kMaxPrimesCrt = 128;
 // ---------------------------------------------------------------------------
 constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
 
-// FP64 products are formed as fma(a, b, 0.0): on gfx950 v_mul_f64 costs more
-// issue time than v_fma_f64 (the same exact butterfly runs 17-23% faster with
-// the FMA in the butterfly probe).  fma(a, b, +0.0) is a * b rounded once,
-// the same double, except that a zero product is +0 where a * b may give -0;
-// every value here ends up as an integer (a conversion, a comparison with 0
-// or a sum with a non-zero integer), where the two zeros are the same.  The
-// compiler keeps the FMA (folding it into a multiply would change the sign of
-// zero) and encodes the 0.0 inline: no register.
-__device__ __forceinline__ double dmul(double a, double b) { return fma(a, b, 0.0); }
-
 __device__ __forceinline__ double mul_mod(double x, double w, double p, double pinv) {
-    const double h = dmul(x, w);
+    const double h = x * w;
     const double l = fma(x, w, -h);
     const double q = fma(h, pinv, kRound) - kRound;
     return fma(-q, p, h) + l;
@@ -143,7 +133,7 @@ __device__ __forceinline__ void gs_bfly(double& x, double& y, double w, double w
 // The twiddle's quotient factor for the five-operation form (unused otherwise).
 template <bool FAST>
 __device__ __forceinline__ double tw_quot(double w, double pinv) {
-    return FAST ? dmul(w, pinv) : 0.0;
+    return FAST ? w * pinv : 0.0;
 }
 
 // K transform levels on the thread's 16 registers, which hold (16 >> K)
@@ -382,7 +372,7 @@ __device__ __forceinline__ void enc_compute(const DevTables& t, uint32_t* __rest
                 // qd = floor(u / p) or one less, so u - qd p (exact: both below 2^47)
                 // is in [0, 2p).
                 const double ud = fma(double(hi), 65536.0, double(lo)) + off[g];
-                const double qd = __builtin_trunc(dmul(ud, inv[g]));
+                const double qd = __builtin_trunc(ud * inv[g]);
                 const uint32_t r = uint32_t(fma(-qd, pd[g], ud));
                 const uint32_t j = pt * 8 + 2 * g + kh;
                 if (j < t.P) out[j * t.N + f] = min(r, r - uint32_t(pd[g]));
@@ -1248,7 +1238,7 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
                     const uint32_t i = i0 + k;
                     if (i < L) {
                         const double tt = sv[k] + carry;
-                        carry = floor(dmul(tt, 0x1p-32));
+                        carry = floor(tt * 0x1p-32);
                         d[i * kCrtDigitStride] = uint32_t(fma(-carry, 4294967296.0, tt));
                     }
                 }
@@ -1267,7 +1257,7 @@ __device__ __forceinline__ void crt_wg(const DevTables& t, const uint32_t* __res
                     if (i < L) {
                         const double qi = double(uint32_t(__builtin_amdgcn_readlane(int(qv), int(i))));
                         const double tt = fma(-Q, qi, sv[k]) + carry;
-                        carry = floor(dmul(tt, 0x1p-32));
+                        carry = floor(tt * 0x1p-32);
                         d[i * kCrtDigitStride] = uint32_t(fma(-carry, 4294967296.0, tt));
                     }
                 }

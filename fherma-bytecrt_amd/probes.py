@@ -101,27 +101,54 @@ def measure_hbm(torch, nbytes: int = 2 << 30, reps: int = 5) -> dict:
     return out
 
 
-def measure_butterflies(torch, p: int, tw15, threads: int = 1 << 20, rounds: int = 64, reps: int = 5) -> dict:
+def measure_butterflies(torch, p: int, tw15, threads: int = 1 << 20, rounds: int = 64, reps: int = 5,
+                        warmup_s: float = 0.5) -> dict:
     """Butterflies per second (x 1e12) of each form, with as many waves per
     SIMD as registers allow and (``*_4w``) with four, and whether all forms
-    produced the same residues.  The five-operation forms need p - 1 = k 2^s
-    with 3 p k < 2^53 (true for the primes of N = 65,536); for other primes
-    they are left out."""
+    produced the same residues.  Every variant runs round robin, first for
+    ``warmup_s`` untimed, then ``reps`` timed rounds in alternating order,
+    and each rate is the median of its rounds: timing the forms one after
+    the other let the clock, still rising after the work before, favour
+    whichever came later (by up to 20%).  The five-operation forms need
+    p - 1 = k 2^s with 3 p k < 2^53 (true for the primes of N = 65,536); for
+    other primes they are left out."""
+    import time
+
     stream = torch.cuda.Stream()
-    outs, res = {}, {"threads": threads, "rounds": rounds, "p": p}
+    res = {"threads": threads, "rounds": rounds, "p": p}
     k = p - 1
     while k % 2 == 0:
         k //= 2
-    for form, name in FORMS.items():
-        if form in (3, 4) and 3 * p * k >= 1 << 53:
-            continue
-        for lds, suffix in ((0, ""), (LDS_FOUR_WAVES, "_4w")):
-            out = torch.empty(threads * 16, dtype=torch.int32, device="cuda")
-            ms = _timed(torch, stream, lambda: butterflies(form, out, threads, p, tw15, rounds, stream, lds), reps)
-            res[f"{name}{suffix}_Tbfly_per_s"] = round(threads * rounds * 32 / (ms * 1e-3) / 1e12, 3)
-            outs[name + suffix] = out
+    variants = [(form, lds) for form in FORMS if not (form in (3, 4) and 3 * p * k >= 1 << 53)
+                for lds in (0, LDS_FOUR_WAVES)]
+    outs = {v: torch.empty(threads * 16, dtype=torch.int32, device="cuda") for v in variants}
+
+    def launch(v):
+        butterflies(v[0], outs[v], threads, p, tw15, rounds, stream, v[1])
+
+    t0 = time.perf_counter()
+    while True:
+        for v in variants:
+            launch(v)
+        stream.synchronize()
+        if time.perf_counter() - t0 >= warmup_s:
+            break
+    times = {v: [] for v in variants}
+    for r in range(reps):
+        for v in (variants if r % 2 == 0 else variants[::-1]):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            launch(v)
+            b.record(stream)
+            b.synchronize()
+            times[v].append(a.elapsed_time(b))
+    for (form, lds), ts in times.items():
+        ts.sort()
+        ms = ts[len(ts) // 2]
+        res[f"{FORMS[form]}{'_4w' if lds else ''}_Tbfly_per_s"] = round(threads * rounds * 32 / (ms * 1e-3) / 1e12, 3)
     torch.cuda.synchronize()
-    res["forms_agree"] = all(torch.equal(outs["fp64"], o) for o in outs.values())
+    ref = outs[(0, 0)]
+    res["forms_agree"] = all(torch.equal(ref, o) for o in outs.values())
     return res
 
 

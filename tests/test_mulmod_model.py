@@ -10,7 +10,7 @@ levels add at most 8 x 0.63 p), that the result is exactly x w mod p and at
 most 0.63 p in magnitude, and that balanced() returns |r| <= p/2 + 1.
 The NTT kernels' five-operation twiddle product (mul_mod_tw) is modelled on
 the plans' own primes within its bound |q| k < 2^53 (p - 1 = k 2^s), and shown
-to fail past it; dmul(a, b) = fma(a, b, 0.0) is shown to be the product.
+to fail past it.
 """
 from fractions import Fraction
 
@@ -83,11 +83,7 @@ def test_canon_product_of_mul_mod():
 # The NTT kernels' five-operation twiddle product (bytecrt_kernels.hip,
 # mul_mod_tw): for p - 1 = k 2^s and w in [0, p), with wq = fl(w / p),
 #   q = rint(x wq),  r = fma(x, w, -(q p - q)) - q  (q p - q by one FMA),
-# exact while |q| k < 2^53.  The kernels form products as fma(a, b, 0.0).
-
-def dmul(a: float, b: float) -> float:
-    return fma(a, b, 0.0)
-
+# exact while |q| k < 2^53.
 
 def mul_mod_tw(x: float, w: float, wq: float, p: float) -> float:
     q = fma(x, wq, K_ROUND) - K_ROUND
@@ -118,7 +114,7 @@ def plan_primes():
 def test_mul_mod_tw_exact_within_its_bound():
     """Exact, with |r| <= p/2 + 2^-13 p, for every |x| whose quotient bound
     (|x| + 1) k stays below 2^53 (capped at 2^38, the kernels' largest input),
-    on the plans' own primes; dmul(w, pinv) is the kernels' wq."""
+    on the plans' own primes; w * pinv is the kernels' wq."""
     rng = np.random.default_rng(55)
     for N, p_int in plan_primes():
         k = odd_part(p_int - 1)
@@ -130,7 +126,7 @@ def test_mul_mod_tw_exact_within_its_bound():
         for x_int in xs:
             for w_int in (int(rng.integers(0, p_int)), p_int - 1, 1, 0, p_int // 2):
                 w = float(w_int)
-                r = mul_mod_tw(float(x_int), w, dmul(w, pinv), p)
+                r = mul_mod_tw(float(x_int), w, w * pinv, p)
                 assert r == int(r)
                 assert (int(r) - x_int * w_int) % p_int == 0, (N, p_int, x_int, w_int)
                 assert abs(r) <= p_int / 2 + p_int * 2.0 ** -13
@@ -152,19 +148,7 @@ def test_mul_mod_tw_bound_is_needed():
             x_int = int(rng.integers(lo, 4 * lo))
             w_int = int(rng.integers(p_int // 2, p_int))
             w = float(w_int)
-            r = mul_mod_tw(float(x_int), w, dmul(w, pinv), p)
+            r = mul_mod_tw(float(x_int), w, w * pinv, p)
             if (int(r) - x_int * w_int) % p_int:
                 bad += 1
     assert bad > 0
-
-
-def test_fma_product_is_the_product():
-    """dmul(a, b) = fma(a, b, 0.0) is fl(a b) for every pair of integers the
-    kernels multiply, apart from the sign of a zero product."""
-    rng = np.random.default_rng(57)
-    for _ in range(2000):
-        a = float(int(rng.integers(-(1 << 48), 1 << 48)))
-        b = float(int(rng.integers(0, 1 << 31)))
-        assert dmul(a, b) == a * b
-        assert dmul(a, 1.0 / 2147352577.0) == a * (1.0 / 2147352577.0)
-    assert dmul(-3.0, 0.0) == 0.0  # +0 where -3 * 0 = -0: equal as numbers
