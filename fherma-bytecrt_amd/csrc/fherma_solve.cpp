@@ -14,11 +14,14 @@
 #include <atomic>
 #include <cerrno>
 #include <climits>
+#include <condition_variable>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
 #include "../../include/bytecrt.h"
+#include "bytecrt_copy.hpp"
 #include "bytecrt_trace.hpp"
 
 #ifndef MADV_POPULATE_WRITE
@@ -30,8 +33,13 @@ namespace {
 struct Solver {
     bcrt_engine* engine = nullptr;
     uint32_t B = 0, N = 0, L = 0;
+    std::unique_ptr<bcrt::CopyPool> out_pool;  // batched: copies finished pieces into the owned vector
     ~Solver() { bcrt_destroy(engine); }
 };
+
+// Owned outputs of at least this many bytes are filled in parallel (Grower);
+// smaller ones by appends.
+constexpr size_t kParallelOutputBytes = size_t(64) << 20;
 
 [[noreturn]] void raise(int code, const std::string& msg) {
     if (code == BCRT_EINVAL) throw std::invalid_argument(msg);
@@ -83,6 +91,68 @@ void append(void* ctx, const uint32_t* words, size_t nwords) {
     v->insert(v->end(), words, words + nwords);
 }
 
+// Grows the owned output vector, whose storage is reserved for the whole
+// result, in steps ahead of the finished pieces: a resize within the
+// capacity value-initializes (zero-fills) the new elements and never moves
+// the storage, and once a step is published the pieces below it are copied
+// in by several threads at once, instead of one thread appending them (the
+// append had been the longest lane of a B=128 call).  Only this thread
+// touches the vector object; the copies write its elements through a
+// pointer taken before the thread starts, and only below the published size.
+class Grower {
+   public:
+    Grower(std::vector<uint32_t>& v, size_t total) : v_(v), total_(total) {
+        th_ = std::thread([this] { run(); });
+    }
+    ~Grower() {
+        th_.join();  // always reaches total: the vector then has its full size
+    }
+    // Waits until the first n elements exist.
+    void wait_for(size_t n) {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return ready_ >= n; });
+    }
+
+   private:
+    static constexpr size_t kStep = (size_t(16) << 20) / sizeof(uint32_t);  // 16 MiB per resize
+    void run() {
+        bcrt::TraceScope tr("bcrt:grow");
+        size_t n = v_.size();
+        while (n < total_) {
+            n = std::min(total_, n + kStep);
+            v_.resize(n);
+            {
+                std::lock_guard<std::mutex> g(m_);
+                ready_ = n;
+            }
+            cv_.notify_all();
+        }
+    }
+    std::vector<uint32_t>& v_;
+    const size_t total_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    size_t ready_ = 0;
+    std::thread th_;
+};
+
+// Sink of a batched call: each finished piece, once the vector has grown
+// past it, is copied into place by the solver's copy pool.
+struct ParallelOut {
+    uint32_t* base;
+    size_t off;
+    Grower* grower;
+    bcrt::CopyPool* pool;
+};
+
+void place(void* ctx, const uint32_t* words, size_t nwords) {
+    auto* o = static_cast<ParallelOut*>(ctx);
+    o->grower->wait_for(o->off + nwords);
+    bcrt::TraceScope tr("bcrt:place");
+    o->pool->copy(o->base + o->off, words, nwords * sizeof(uint32_t));
+    o->off += nwords;
+}
+
 }  // namespace
 
 void* fherma_init(const fherma::Point& p) {
@@ -105,6 +175,8 @@ void* fherma_init(const fherma::Point& p) {
     s->B = p.B;
     s->N = p.N;
     s->L = p.L;
+    if (size_t(p.B) * p.N * p.L * sizeof(uint32_t) >= kParallelOutputBytes)
+        s->out_pool = std::make_unique<bcrt::CopyPool>(bcrt::pool_workers());
     return s.release();
 }
 
@@ -120,7 +192,15 @@ fherma::Outputs fherma_run(void* state, const fherma::Inputs& in) {
     int rc;
     {
         Prefault pages(out.c.data.data(), total * sizeof(uint32_t));
-        rc = bcrt_run_sink(s.engine, in.a.data.data(), in.b.data.data(), s.B, append, &out.c.data);
+        if (s.out_pool) {
+            // the pointer is taken before the grower thread starts
+            ParallelOut o{out.c.data.data(), 0, nullptr, s.out_pool.get()};
+            Grower grow(out.c.data, total);
+            o.grower = &grow;
+            rc = bcrt_run_sink(s.engine, in.a.data.data(), in.b.data.data(), s.B, place, &o);
+        } else {
+            rc = bcrt_run_sink(s.engine, in.a.data.data(), in.b.data.data(), s.B, append, &out.c.data);
+        }
     }
     if (rc != BCRT_OK) raise(rc, bcrt_last_error(s.engine));
     if (out.c.data.size() != total) throw std::runtime_error("ByteCRT output coefficient count");
