@@ -21,7 +21,8 @@
 // balanced range and every modular product is exact (see mul_mod).  Between
 // kernels the encode output and the CRT terms are canonical u32 in [0, p);
 // the forward-column and middle outputs, read only by the next NTT kernel,
-// are balanced int32 (|r| <= p/2 + 1), which is cheaper to form.
+// are balanced int32 (|r| <= p/2 + 1; half of the middle kernel's words are
+// its last modular products, |r| <= p/2 + 2^-12 p), which is cheaper to form.
 #include "bytecrt_kernels.hpp"
 
 #include <algorithm>
@@ -507,11 +508,11 @@ __device__ __forceinline__ void fwd_cols_wg(const DevTables& t, uint32_t* __rest
     }
 }
 
-// Inverse columns take balanced words (|x| <= p/2 + 1 < m0 = p/2 + 2^-13 p);
-// each level at most doubles the largest value, so the inputs of the last
-// (folded) level are below 2^n1 m0, |q| < 2^(n1-1) p (1 + 2^-12) + 1, and
-// with s >= n1 + 9, |q| k < p^2 (1 + 2^-12) / 2^10 + 2^22 < 2^53 (FAST
-// holds for every plan).
+// Inverse columns take the middle kernel's words: balanced (|x| <= p/2 + 1)
+// or its last modular products (|x| <= m0 = p/2 + 2^-12 p); each level at
+// most doubles the largest value, so the inputs of the last (folded) level
+// are below 2^n1 m0, |q| < 2^(n1-1) p (1 + 2^-11) + 1, and with s >= n1 + 9,
+// |q| k < p^2 (1 + 2^-11) / 2^10 + 2^22 < 2^53 (FAST holds for every plan).
 template <int NLOG1, bool FAST = true>
 __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __restrict__ planes, uint32_t bx,
                                             uint32_t prod, double* s) {
@@ -727,10 +728,13 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
     gs16_tw<F2>(v, twb, p, pinv);
     __syncthreads();
     {
-        // balanced residues back in place, then row u out as four 16-byte stores
+        // balanced residues back in place, then row u out as four 16-byte stores;
+        // v[8..15] come straight from the last level's modular products
+        // (|r| <= p/2 + 2^-12 p, below 2^31), so they only need converting
         uint32_t* sy = reinterpret_cast<uint32_t*>(sx);
 #pragma unroll
-        for (int x = 0; x < 16; ++x) sy[bl * kMidS + skew(tau + 16 * x)] = uint32_t(balanced(v[x], p, pinv));
+        for (int x = 0; x < 16; ++x)
+            sy[bl * kMidS + skew(tau + 16 * x)] = uint32_t(x < 8 ? balanced(v[x], p, pinv) : int32_t(v[x]));
         __syncthreads();
         const uint32_t u = threadIdx.x;
         uint4* dst = reinterpret_cast<uint4*>(pa + size_t(u) * t.N1 + b0);
