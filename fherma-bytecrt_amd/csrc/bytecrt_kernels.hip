@@ -4,7 +4,7 @@
 //   k_encode      wide coefficients -> residues mod each prime (int8 MFMA byte-power GEMM)
 //   k_fwd_cols    forward NTT levels 0..n1-1 (columns of N1 elements, stride N2)
 //   k_middle      forward levels n1..n-1 of A and B, pointwise product, inverse levels n-1..n1
-//   k_inv_cols    inverse levels n1-1..0, scale by N^-1 * (P/p)^-1 (CRT terms)
+//   k_inv_cols    inverse levels n1-1..0, scale by (N/2)^-1 * (P/p)^-1 (CRT terms)
 //   k_crt         centered alpha + int8 MFMA CRT GEMM + carry + generic mod-q reduction
 // For N >= 16,384 (N1 >= 64) the product path runs k_encode_fwd, one kernel
 // for the first two (a workgroup per 256 coefficients of whole columns); the
@@ -91,7 +91,8 @@ __device__ __forceinline__ double word_value(uint32_t w) { return double(int32_t
 //       and |wq - w/p| <= 2^-52, so |q - x w / p| <= 1/2 + |x| 2^-52;
 //   c = q p - q = q (p - 1) = (q k) 2^s by one FMA: exact while |q| k < 2^53;
 //   t = x w - c = (x w - q p) + q: one FMA, exact (an integer below 2^40);
-//   r = t - q = x w - q p, exact, with |r| <= p/2 + 2^-13 p for |x| < 2^38.
+//   r = t - q = x w - q p, exact, with |r| <= p/2 + |x| 2^-52 p, which is
+//       below p/2 + 2^-13 p for |x| < 2^39.
 // No TwoProduct is needed: the product's low part never has to be rounded.
 // Each caller states the largest |x| it passes and why |q| k < 2^53 holds
 // there (|q| < |x| + 1); mul_mod stays where inputs reach 2^48 (the fused
@@ -565,21 +566,24 @@ __device__ __forceinline__ void inv_cols_wg(const DevTables& t, uint32_t* __rest
 // Twiddles of a radix-16 round (K = 4, one sub-block): tw[2^l - 1 + g] is the
 // twiddle of group g at local level l, loaded ahead of the round that uses it
 // and widened to FP64 there (u32 halves the registers they hold meanwhile).
+// (LEVELS < 4 loads only the round's first LEVELS levels.)
+template <int LEVELS = 4>
 __device__ __forceinline__ void load_tw16(uint32_t (&tw)[15], const uint32_t* __restrict__ z, uint32_t mult,
                                           uint32_t e0, uint32_t o0) {
 #pragma unroll
-    for (int l = 0; l < 4; ++l)
+    for (int l = 0; l < LEVELS; ++l)
 #pragma unroll
         for (int g = 0; g < (1 << l); ++g) tw[(1 << l) - 1 + g] = z[(mult << (e0 + l)) + (o0 << l) + g];
 }
 
 // Forward radix-16 round on two operands of the same prime and block at once:
-// each twiddle serves two independent butterfly chains.
-template <bool FAST>
+// each twiddle serves two independent butterfly chains.  (LEVELS = 3 leaves
+// the last level to mid_pair_step.)
+template <bool FAST, int LEVELS = 4>
 __device__ __forceinline__ void ct16x2_tw(double (&v)[16], double (&u)[16], const uint32_t (&tw)[15], double p,
                                           double pinv) {
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
+    for (int l = 0; l < LEVELS; ++l) {
         const int half = 1 << (3 - l);
 #pragma unroll
         for (int g = 0; g < (1 << l); ++g) {
@@ -594,10 +598,12 @@ __device__ __forceinline__ void ct16x2_tw(double (&v)[16], double (&u)[16], cons
     }
 }
 
-template <bool FAST>
+// Inverse radix-16 round from local level TOP down (TOP = 2: the round's
+// first level was done by mid_pair_step).
+template <bool FAST, int TOP = 3>
 __device__ __forceinline__ void gs16_tw(double (&v)[16], const uint32_t (&tw)[15], double p, double pinv) {
 #pragma unroll
-    for (int l = 3; l >= 0; --l) {
+    for (int l = TOP; l >= 0; --l) {
         const int half = 1 << (3 - l);
 #pragma unroll
         for (int g = 0; g < (1 << l); ++g) {
@@ -611,9 +617,39 @@ __device__ __forceinline__ void gs16_tw(double (&v)[16], const uint32_t (&tw)[15
     }
 }
 
+// The last forward level, the pointwise product and the first inverse level
+// of each pair (2g, 2g + 1), in one step.  With the pair's forward twiddle w
+// (its inverse twiddle is w^-1), those three levels map (a0, a1), (b0, b1) to
+//   2 (a0 b0 + w^2 a1 b1)  and  2 (a0 b1 + a1 b0).
+// This returns both without the factor 2 (the plan's inverse scale is
+// (N/2)^-1 for it), the second as (a0 + a1)(b0 + b1) - a0 b0 - a1 b1: three
+// products where the levels take four twiddle and two pointwise products.
+// w^2 needs no table: z[k] = psi^brv(k) has z[2m + b]^2 = (-1)^b z[m], so
+// the square of group g's twiddle is (-1)^g times twiddle tw[3 + g/2] of the
+// level before.  Inputs |a|, |b| < 4.01 p, so the sums' products stay below
+// 2^68, where mul_mod is exact; the outputs are below 2 m and 3 m, m the
+// largest modular product.
+template <bool FAST>
+__device__ __forceinline__ void mid_pair_step(double (&v)[16], const double (&u)[16], const uint32_t (&tw)[15],
+                                              double p, double pinv) {
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+        const double a0 = v[2 * g], a1 = v[2 * g + 1], b0 = u[2 * g], b1 = u[2 * g + 1];
+        const double c00 = mul_mod(a0, b0, p, pinv);
+        const double c11 = mul_mod(a1, b1, p, pinv);
+        const double cs = mul_mod(a0 + a1, b0 + b1, p, pinv);
+        const double w2 = double(tw[3 + (g >> 1)]);
+        const double t = tw_mul<FAST>(c11, w2, tw_quot<FAST>(w2, pinv), p, pinv);
+        v[2 * g] = (g & 1) ? c00 - t : c00 + t;
+        v[2 * g + 1] = cs - c00 - c11;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Middle: 16 blocks (256 contiguous natural coefficients each) of one prime,
-// forward levels n1..n-1 for A and B, pointwise product, inverse levels back.
+// forward levels n1..n-1 for A and B, pointwise product, inverse levels back
+// (the last forward level, the product and the first inverse level as one
+// pair step).
 // Grid (N1/16, batch, P): workgroups of one prime run together so its block
 // twiddles stay in L2 across the whole batch; each radix-16 round's 15
 // twiddles are loaded while the round before it computes.  One exchange
@@ -622,15 +658,16 @@ __device__ __forceinline__ void gs16_tw(double (&v)[16], const uint32_t (&tw)[15
 //
 // Twiddle products (MODE): 0 mul_mod throughout; 1 the five-operation form
 // except in the last four inverse levels; 2 the five-operation form
-// throughout.  Bounds, with m0 = p/2 + 2^-13 p the largest modular product:
+// throughout.  Bounds, with m0 = p/2 + 2^-12 p the largest modular product:
 // the forward levels start from balanced words and add at most m0 per level
-// (inputs < p/2 + 1 + 7 m0 < 4.01 p); the pointwise product (mul_mod)
-// returns at most m0;
-// inverse level i (i = 1 .. 8 after the product) takes inputs below 2^i m0.
-// So |q| k < 8.01 p 2^18 < 2^53 for the forward levels and for i <= 4 and
-// every plan (s >= 13), while
-// the last four levels reach |q| < 128 p + p / 32 + 1, which the plan checks
-// prime by prime (kPlanTw5Late; always true for N = 65,536, where s >= 17).
+// (the pair step's inputs < p/2 + 1 + 7 m0 < 4.01 p); the pair step returns
+// words below 2 m0 and 3 m0, and each later inverse level at most doubles
+// them, so the differences that inverse level i (i = 2 .. 8, the pair step
+// being level 1) multiplies are below 3 m0 2^(i-1).  So |q| k < 12.01 p 2^18
+// < 2^53 for the forward levels and for i <= 4 and every plan (s >= 13),
+// while the last four levels reach |x| < 384 m0 < 192 p + 3 p / 32 (< 2^39),
+// which the plan checks prime by prime (kPlanTw5Late; always true for
+// N = 65,536, where s >= 17).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMidS = exchange_stride<256>();
 
@@ -695,7 +732,7 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
         v[x] = word_value(ra[skew(tau + 16 * x)]);
         w[x] = word_value(rb[skew(tau + 16 * x)]);
     }
-    load_tw16(twb, zf, mult, 4, tau);
+    load_tw16<3>(twb, zf, mult, 4, tau);
     ct16x2_tw<F1>(v, w, twa, p, pinv);
     // Exchange A, then B, as FP64 through the block's row: no reduction (the
     // values stay below p/2 + 1 + 4 m0 < 2.51 p) and no conversions.
@@ -709,16 +746,16 @@ __device__ __forceinline__ void middle_wg(const DevTables& t, uint32_t* __restri
 #pragma unroll
     for (int x = 0; x < 16; ++x) rx[skew(tau + 16 * x)] = w[x];
     wave_lds_sync();
-    // sub-levels 4..7 (positions 16 tau + x)
+    // sub-levels 4..6 (positions 16 tau + x), then sub-level 7, the
+    // pointwise product and inverse sub-level 7 as one pair step
 #pragma unroll
     for (int x = 0; x < 16; ++x) w[x] = rx[skew(16 * tau + x)];
-    load_tw16(twa, zi, mult, 4, tau);
-    ct16x2_tw<F1>(v, w, twb, p, pinv);
-    // pointwise product (|v|, |w| < 2.51 p + 4 m0 < 2^33.2) and inverse sub-levels 7..4
-#pragma unroll
-    for (int x = 0; x < 16; ++x) v[x] = mul_mod(v[x], w[x], p, pinv);
+    load_tw16<3>(twa, zi, mult, 4, tau);
+    ct16x2_tw<F1, 3>(v, w, twb, p, pinv);
+    mid_pair_step<F1>(v, w, twb, p, pinv);
+    // inverse sub-levels 6..4
     load_tw16(twb, zi, mult, 0, 0);
-    gs16_tw<F1>(v, twa, p, pinv);
+    gs16_tw<F1, 2>(v, twa, p, pinv);
     wave_lds_sync();  // every lane of the wave has read B from the row
 #pragma unroll
     for (int x = 0; x < 16; ++x) rx[skew(16 * tau + x)] = v[x];

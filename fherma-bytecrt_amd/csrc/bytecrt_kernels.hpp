@@ -19,7 +19,7 @@ struct DevTables {
     const double* pd;              // [P][2]: p, 1/p
     const uint32_t* zf;            // [P][N]: psi^brv(k) mod p
     const uint32_t* zi;            // [P][N]: psi^-brv(k) mod p
-    const double* scale;           // [P]: N^-1 (P/p)^-1 mod p
+    const double* scale;           // [P]: (N/2)^-1 (P/p)^-1 mod p
     const double* scale_z1;        // [P]: scale psi^-brv(1) mod p
     const double* zf16;            // [P][16]: psi^brv(k) mod p, k < 16
     const void* enc_prime;         // [P] x 24 bytes: off (f64), 1/p rounded down (f64), p, off mod 2^32
@@ -42,7 +42,7 @@ constexpr uint32_t kFlagMulModTw6 = 64;    // NTT kernels: six-operation twiddle
 constexpr uint32_t kFlagsAll = 127;
 // Set by the engine from its plan, not a choice: every prime allows the
 // five-operation twiddle products in the middle kernel's last four inverse
-// levels (|q| k < 2^53 for |q| < 128 p + p / 32 + 1, where p - 1 = k 2^s).
+// levels (|q| k < 2^53 for |q| < 192 p + 3 p / 32 + 1, where p - 1 = k 2^s).
 constexpr uint32_t kPlanTw5Late = 1u << 31;
 constexpr uint32_t kPlanFlags = kPlanTw5Late;
 

@@ -262,12 +262,12 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
     for (uint32_t j = 0; j < P; ++j) {
         const uint32_t p = pl.primes[j];
         // The middle kernel's last four inverse levels pass inputs up to
-        // 128 p + p / 32 to twiddle products whose quotient q times the odd
+        // 192 p + 3 p / 32 to twiddle products whose quotient q times the odd
         // part k of p - 1 must stay below 2^53 (exact q (p - 1)).
         {
             uint32_t k = p - 1;
             while (!(k & 1)) k >>= 1;
-            const unsigned __int128 qmax = uint64_t(128) * p + p / 32 + 2;
+            const unsigned __int128 qmax = uint64_t(192) * p + uint64_t(3) * p / 32 + 2;
             if (qmax * k >= (unsigned __int128)1 << 53) pl.tw5_late = false;
         }
         // Smallest base a >= 2 with (a^((p-1)/2N))^N == -1 (rns-plan.hpp:163-170).
@@ -288,8 +288,10 @@ HostPlan make_plan(uint32_t N, uint32_t W, uint32_t L, const uint32_t* qw) {
 
         pl.pd[2 * j] = double(p);
         pl.pd[2 * j + 1] = 1.0 / double(p);
-        // The inverse transform leaves N * c; t_j = c * crt_inverse_j.
-        const uint32_t n_inv = pow_mod(N % p, p - 2, p);
+        // The inverse transform leaves (N / 2) c (the middle kernel's pair
+        // step, which merges the last forward level, the pointwise product
+        // and the first inverse level, drops one factor 2); t_j = c crt_inverse_j.
+        const uint32_t n_inv = pow_mod((N / 2) % p, p - 2, p);
         pl.scale[j] = double(mulm(n_inv, pl.crt_inverse[j], p));
 
         // Twiddles: zf[k] = psi^brv(k), zi[k] = psi^-brv(k) (log_n-bit reversal).

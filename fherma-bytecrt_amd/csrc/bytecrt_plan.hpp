@@ -46,7 +46,7 @@ struct HostPlan {
     std::vector<uint32_t> crt_inverse;  // P: (P/p_j)^-1 mod p_j
     std::vector<uint32_t> weights;      // (P+1) x L: P_j mod q, last row (-P) mod q
     std::vector<double> pd;             // P x 2: (p, 1/p) for the FP64 modular products
-    std::vector<double> scale;          // P: N^-1 * crt_inverse mod p
+    std::vector<double> scale;          // P: (N/2)^-1 * crt_inverse mod p
     std::vector<double> scale_z1;       // P: scale * psi^-brv(1) mod p (last inverse level, scale folded in)
     std::vector<double> zf16;           // P x 16: psi^brv(k) mod p for k < 16 (first forward column round)
     std::vector<uint32_t> zf, zi;       // P x N: psi^{+-brv(k)} mod p
@@ -61,7 +61,7 @@ struct HostPlan {
     uint64_t qt1 = 0;                   // (q >> qshift) + 1
     double inv_qt1 = 0;                 // 1 / qt1
     uint32_t gap = 0;                   // 2^W - q when below 2^32 and L >= 3 (near-power CRT reduction), else 0
-    bool tw5_late = false;              // every prime: |q| k < 2^53 for |q| < 128 p + p / 32 + 1 (p - 1 = k 2^s)
+    bool tw5_late = false;              // every prime: |q| k < 2^53 for |q| < 192 p + 3 p / 32 + 1 (p - 1 = k 2^s)
 };
 
 // Throws std::invalid_argument (bad geometry or q) or std::runtime_error (no plan).

@@ -56,7 +56,7 @@ def test_ntt_stages_dense_protocol(pkg, orc, gu, N, W, tw6):
     """ntt-smoke.cu protocol: dense splitmix inputs with boundary words per prime;
     the three NTT stages leave t_j = (a*b mod (X^N+1, p_j)) (P/p_j)^-1 mod p_j.
     Both twiddle-product forms: the five-operation one (the default; at N <=
-    16,384 the middle kernel's last four inverse levels keep mul_mod, its
+    32,768 the middle kernel's last four inverse levels keep mul_mod, its
     plan check fails there) and mul_mod throughout (FLAG_MULMOD_TW6)."""
     q = pkg.datagen.modulus(W, 3)
     e = pkg.Engine(N, W, q)

@@ -113,13 +113,14 @@ def plan_primes():
 
 def test_mul_mod_tw_exact_within_its_bound():
     """Exact, with |r| <= p/2 + 2^-13 p, for every |x| whose quotient bound
-    (|x| + 1) k stays below 2^53 (capped at 2^38, the kernels' largest input),
-    on the plans' own primes; w * pinv is the kernels' wq."""
+    (|x| + 1) k stays below 2^53 (capped at 2^39, above the kernels' largest
+    input 192 p + 3 p / 32), on the plans' own primes; w * pinv is the
+    kernels' wq."""
     rng = np.random.default_rng(55)
     for N, p_int in plan_primes():
         k = odd_part(p_int - 1)
         assert (p_int - 1) % (2 * N) == 0
-        xmax = min((1 << 53) // k - 2, 1 << 38)
+        xmax = min((1 << 53) // k - 2, (1 << 39) - 1)
         p, pinv = float(p_int), 1.0 / float(p_int)
         xs = [int(v) for v in rng.integers(-xmax, xmax + 1, size=250)]
         xs += [xmax, -xmax, 0, 1, -1, p_int - 1, -(p_int - 1)]
@@ -152,3 +153,81 @@ def test_mul_mod_tw_bound_is_needed():
             if (int(r) - x_int * w_int) % p_int:
                 bad += 1
     assert bad > 0
+
+
+# ---------------------------------------------------------------------------
+# The middle kernel's pair step (bytecrt_kernels.hip, mid_pair_step): the last
+# forward level, the pointwise product and the first inverse level of a pair
+# in one step, with w^2 taken from the level before's twiddle.
+
+def brv(k: int, bits: int) -> int:
+    return int(format(k, f"0{bits}b")[::-1], 2)
+
+
+def test_twiddle_square_is_parent_twiddle():
+    """z[k] = psi^brv(k) (log2 N-bit reversal) has z[2m + b]^2 = (-1)^b z[m]:
+    checked for every k at N = 4,096 and on a sample at N = 65,536, with the
+    oracle's psi."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+    import oracle
+    rng = np.random.default_rng(7)
+    for N, W, ks in ((4096, 100, None), (65536, 1747, 3000)):
+        pl = oracle.Plan(N, W, (1 << (W - 1)) | 1)
+        n = N.bit_length() - 1
+        for j in (0, len(pl.primes) - 1):
+            p, psi = int(pl.primes[j]), int(pl.psi[j])
+            z = lambda k: pow(psi, brv(k, n), p)
+            sample = range(2, N) if ks is None else (int(k) for k in rng.integers(2, N, size=ks))
+            for k in sample:
+                want = z(k >> 1) if k % 2 == 0 else (p - z(k >> 1)) % p
+                assert z(k) * z(k) % p == want, (N, p, k)
+
+
+def test_pair_step_equals_three_levels():
+    """For the plan's primes and twiddles of the last level (k = N/2 + g), on
+    inputs up to the pair step's bound 4.01 p: the FP64 model of the step
+    returns exactly half of what the forward butterfly, the pointwise products
+    and the inverse butterfly give (mod p), as its outputs (a0 b0 +/- w'
+    a1 b1, (a0 + a1)(b0 + b1) - a0 b0 - a1 b1) with w' the parent twiddle and
+    the sign (-1)^g; and |out0| < 2 m, |out1| < 3 m for m = p/2 + 2^-12 p."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+    import oracle
+    rng = np.random.default_rng(71)
+    N, W = 65536, 1747
+    pl = oracle.Plan(N, W, (1 << (W - 1)) | 1)
+    n = N.bit_length() - 1
+    for j in (0, 57, len(pl.primes) - 1):
+        p_int, psi = int(pl.primes[j]), int(pl.psi[j])
+        p, pinv = float(p_int), 1.0 / float(p_int)
+        half_inv = pow(2, p_int - 2, p_int)
+        m = p_int / 2 + p_int * 2.0 ** -12
+        lim = int(4.01 * p_int)
+        for trial in range(120):
+            g = int(rng.integers(0, N // 2))
+            k = N // 2 + g
+            w = pow(psi, brv(k, n), p_int)
+            w_inv = pow(w, p_int - 2, p_int)
+            parent = pow(psi, brv(k >> 1, n), p_int)
+            if trial < 8:
+                a0, a1, b0, b1 = [lim * (1 if (trial >> i) & 1 else -1) for i in range(3)] + [lim]
+            else:
+                a0, a1, b0, b1 = (int(v) for v in rng.integers(-lim, lim + 1, size=4))
+            # the three levels, exactly
+            fa0, fa1 = (a0 + w * a1) % p_int, (a0 - w * a1) % p_int
+            fb0, fb1 = (b0 + w * b1) % p_int, (b0 - w * b1) % p_int
+            c0, c1 = fa0 * fb0 % p_int, fa1 * fb1 % p_int
+            want0, want1 = (c0 + c1) * half_inv % p_int, (c0 - c1) * w_inv * half_inv % p_int
+            # the FP64 step
+            c00 = mul_mod(float(a0), float(b0), p, pinv)
+            c11 = mul_mod(float(a1), float(b1), p, pinv)
+            cs = mul_mod(float(a0 + a1), float(b0 + b1), p, pinv)
+            w2 = float(parent)
+            t = mul_mod_tw(c11, w2, w2 * pinv, p)
+            out0 = c00 - t if g & 1 else c00 + t
+            out1 = cs - c00 - c11
+            assert int(out0) % p_int == want0 and int(out1) % p_int == want1, (p_int, g, a0, a1, b0, b1)
+            assert abs(out0) < 2 * m and abs(out1) < 3 * m
