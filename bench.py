@@ -635,11 +635,14 @@ def main():
     # N/2 log2 N each) at the measured rates of the forms the kernels use: the
     # fused encode's forward column levels (n1 per operand) the six-operation
     # form, the middle and inverse-column levels the five-operation form
-    # (three and four waves per SIMD, the kernels' occupancy)
+    # (three and four waves per SIMD, the kernels' occupancy).  The middle
+    # kernel's pair step (last forward level of both operands, pointwise
+    # product, first inverse level: 28 FP64 operations per pair) counts as
+    # four butterflies of seven operations, where those three levels had three
     bf = ceilings["butterflies"]
     n1 = (N // 256).bit_length() - 1
     cols_bfly = 2 * P * (N // 2) * n1 * B
-    rest_bfly = 3 * P * (N // 2) * 16 * B - cols_bfly
+    rest_bfly = 3 * P * (N // 2) * 16 * B - cols_bfly + P * (N // 2) * B
     ceilings["ntt_fp64_floor_ms"] = round((cols_bfly / bf["fp64_4w_Tbfly_per_s"] +
                                            rest_bfly / bf["fp64_tw5_4w_Tbfly_per_s"]) / 1e12 * 1e3, 3)
     # NTT-only kernels of the product path (the forward column levels run
