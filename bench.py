@@ -13,6 +13,7 @@ import argparse
 import json
 import os
 import platform
+import statistics
 import sys
 import time
 from pathlib import Path
@@ -266,23 +267,36 @@ def uneven_leg(pkg, torch, steps: int):
     mix = pkg.Mix(groups)
     stream = torch.cuda.Stream()
 
-    def timed(fn):
-        for _ in range(2):
-            fn()
-        torch.cuda.synchronize()
-        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s.record(stream)
-        for _ in range(steps):
-            fn()
-        e.record(stream)
-        torch.cuda.synchronize()
-        return s.elapsed_time(e) / steps
-
-    mixed_ms = timed(lambda: mix.run(stream=stream, graph=True))
+    run_mixed = lambda: mix.run(stream=stream, graph=True)
+    run_separate = lambda: [g[0].run_device(g[1], g[2], g[3], g[4], stream=stream, graph=True) for g in groups]
+    run_mixed()
+    torch.cuda.synchronize()
     got = [g[3].clone() for g in groups]
-    separate_ms = timed(lambda: [g[0].run_device(g[1], g[2], g[3], g[4], stream=stream, graph=True)
-                                 for g in groups])
+    for g in groups:
+        g[3].zero_()
+    run_separate()
+    torch.cuda.synchronize()
     exact = all(bool(torch.equal(g[3], r)) for g, r in zip(groups, got))
+
+    # the two forms timed round robin (alternating which goes first), medians:
+    # timed one after the other, the later form would also gain from a clock
+    # still rising after the work before
+    times = ([], [])
+    for _ in range(2):
+        run_mixed()
+        run_separate()
+    torch.cuda.synchronize()
+    for r in range(6):
+        for i in ((0, 1) if r % 2 == 0 else (1, 0)):
+            fn = run_mixed if i == 0 else run_separate
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record(stream)
+            for _ in range(steps):
+                fn()
+            e.record(stream)
+            torch.cuda.synchronize()
+            times[i].append(s.elapsed_time(e) / steps)
+    mixed_ms, separate_ms = statistics.median(times[0]), statistics.median(times[1])
     one = torch.empty_like(groups[3][1][0])
     groups[3][0].run_device(groups[3][1][5], groups[3][2][5], one, 1, stream=stream)
     torch.cuda.synchronize()
