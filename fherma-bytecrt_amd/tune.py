@@ -180,7 +180,8 @@ def tune_fusion(engine, a, b, c, batch: int, reps: int = 5, stream=None):
 
 
 # every combination of the five performance flags (32, the CRT's generic-q
-# reduction for near-power moduli, is a switch for tests)
+# reduction for near-power moduli, and 64, the six-operation twiddle products
+# throughout, are switches for tests: each only adds FP64 work)
 KERNEL_FLAG_CANDIDATES = tuple(range(32))
 
 
