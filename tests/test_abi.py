@@ -62,7 +62,7 @@ def test_near_power_gap(pkg):
     assert gap_of(64, (1 << 64) - 1) == 0  # two limbs: the generic steps
 
 
-def test_survey_prime_counts(pkg):
+def test_serving_geometry_prime_counts(pkg):
     # N 8192..65536 with W 220/436/868/1747 -> 15/29/57/114 primes.
     for N, W, P in ((8192, 220, 15), (16384, 436, 29), (32768, 868, 57), (65536, 1747, 114)):
         assert pkg.bytecrt.plan_host(N, W, pkg.datagen.modulus(W, 0))["P"] == P
